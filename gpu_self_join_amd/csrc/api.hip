@@ -1,0 +1,432 @@
+// api.hip -- the C ABI (include/gdsjoin.h): handle API and the legacy
+// GDSJoinPy / copyResultIntoPythonArray drop-in symbols.
+//
+// Host driver flow per join (replaces GDSJoinPy main.cu:360-505 and
+// distanceTableNDGridBatches GPU.cu:270-1088):
+//   grid_build (REORDER, extent, keys, radix sort, cells, SoA, runs, tiles)
+//   -> count pass -> exclusive scan of counts (CSR offsets, total)
+//   -> fill pass straight into the caller's CSR buffer.
+// No sort of result pairs, no per-batch host tables: the device CSR is the
+// neighbor table (the reference's neighborTableLookup, structs.h:55-85).
+#include <math.h>
+#include <stdarg.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+#include "grid.h"
+#include "join.h"
+#include "scan.h"
+#include "sortrows.h"
+
+namespace gdsj {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+}  // namespace gdsj
+
+using namespace gdsj;
+
+struct gdsj_handle {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    Grid G;
+    DevBuf counts, offsets, ids, data, scanTmp, status;
+    gdsj_params p{};
+    int mode = 0;
+    double thr = 0;
+    bool have_counts = false;
+    bool have_table = false;
+    uint64_t total = 0;
+    uint32_t *cur_counts = nullptr;
+    uint64_t *cur_offsets = nullptr;
+    gdsj_stats st{};
+    hipEvent_t ev[8] = {};
+};
+
+extern "C" {
+
+const char *gdsj_last_error(void) { return gdsj::g_err; }
+
+int gdsj_params_default(gdsj_params *p) {
+    if (!p) return GDSJ_EINVAL;
+    memset(p, 0, sizeof(*p));
+    // params.h as shipped (params.h:23,40-60): NUMINDEXEDDIM 6, STAMP 0,
+    // REORDER 1, ILP 8, SHORTCIRCUIT 1, QUERYREORDER 1
+    p->k = 0;
+    p->ilp = 8;
+    p->shortcircuit = 1;
+    p->reorder = 1;
+    p->stamp = 0;
+    p->queryreorder = 1;
+    p->filter = GDSJ_FILTER_PLAIN;
+    p->estimator = GDSJ_EST_EXACT;
+    p->sort_neighbors = 0;
+    p->verbose = 0;
+    p->tile_queries = 0;
+    return GDSJ_OK;
+}
+
+float gdsj_eps_threshold_f32(float eps) {
+    if (!(eps >= 0)) return NAN;
+    float t = eps * eps;
+    while (t > 0 && sqrtf(t) > eps) t = nextafterf(t, 0.0f);
+    for (;;) {
+        float u = nextafterf(t, INFINITY);
+        if (std::isinf(u) || !(sqrtf(u) <= eps)) break;
+        t = u;
+    }
+    return t;
+}
+
+double gdsj_eps_threshold_f64(double eps) {
+    if (!(eps >= 0)) return NAN;
+    double t = eps * eps;
+    while (t > 0 && sqrt(t) > eps) t = nextafter(t, 0.0);
+    for (;;) {
+        double u = nextafter(t, INFINITY);
+        if (std::isinf(u) || !(sqrt(u) <= eps)) break;
+        t = u;
+    }
+    return t;
+}
+
+int gdsj_create(int device, gdsj_handle **out) {
+    if (!out) return GDSJ_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    GDSJ_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) {
+        set_error("device %d out of range (%d devices)", device, ndev);
+        return GDSJ_EINVAL;
+    }
+    GDSJ_HIP(hipSetDevice(device));
+    gdsj_handle *h = new gdsj_handle();
+    h->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        set_error("hipStreamCreate: %s", hipGetErrorString(e));
+        delete h;
+        return GDSJ_EDEVICE;
+    }
+    for (int i = 0; i < 8; i++) (void)hipEventCreate(&h->ev[i]);
+    h->stream = h->own;
+    gdsj_params_default(&h->p);
+    *out = h;
+    return GDSJ_OK;
+}
+
+int gdsj_destroy(gdsj_handle *h) {
+    if (!h) return GDSJ_OK;
+    (void)hipSetDevice(h->device);
+    if (h->own) (void)hipStreamSynchronize(h->own);
+    h->G.release();
+    DevBuf *b[] = {&h->counts, &h->offsets, &h->ids, &h->data, &h->scanTmp, &h->status};
+    for (DevBuf *x : b) x->release();
+    for (int i = 0; i < 8; i++)
+        if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
+    if (h->own) (void)hipStreamDestroy(h->own);
+    delete h;
+    return GDSJ_OK;
+}
+
+static int resolve_params(const gdsj_params *in, uint32_t dims, gdsj_params *out) {
+    if (in) *out = *in; else gdsj_params_default(out);
+    if (out->k == 0) out->k = dims < 6 ? dims : 6;
+    if (out->k > dims || out->k > (uint32_t)MAXK) {
+        set_error("NUMINDEXEDDIM k=%u must be <= dims=%u and <= %d", out->k, dims, MAXK);
+        return GDSJ_EINVAL;
+    }
+    if (out->ilp > 64) {
+        set_error("ILP=%u exceeds 64", out->ilp);
+        return GDSJ_EINVAL;
+    }
+    return GDSJ_OK;
+}
+
+static float ms_between(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+    return ms;
+}
+
+static JoinArgs make_args(gdsj_handle *h) {
+    JoinArgs a{};
+    a.xs = h->G.xs.p;
+    a.ld = h->G.ld;
+    a.ids = h->G.ids.as<uint32_t>();
+    a.tiles = h->G.tiles.as<uint4>();
+    a.ntiles = h->G.ntiles;
+    a.runOff = h->G.runOff.as<uint32_t>();
+    a.runs = h->G.runs.as<uint2>();
+    a.thr = h->thr;
+    a.d = h->G.d;
+    a.ilp = h->p.ilp;
+    a.sc = h->p.shortcircuit;
+    a.status = h->status.as<uint32_t>();
+    return a;
+}
+
+static int do_fill(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
+    hipStream_t s = h->stream;
+    JoinArgs a = make_args(h);
+    a.offsets = h->cur_offsets;
+    a.out = d_ids;
+    a.cap = cap;
+    GDSJ_HIP(hipMemsetAsync(h->status.p, 0, 4, s));
+    GDSJ_HIP(hipEventRecord(h->ev[4], s));
+    GDSJ_TRY(join_pass(h->G, a, h->mode, true, s));
+    GDSJ_HIP(hipEventRecord(h->ev[5], s));
+    if (h->p.sort_neighbors)
+        GDSJ_TRY(sort_rows(h->cur_offsets, h->G.n, d_ids, s));
+    GDSJ_HIP(hipEventRecord(h->ev[6], s));
+    uint32_t flag = 0;
+    GDSJ_HIP(hipMemcpyAsync(&flag, h->status.p, 4, hipMemcpyDeviceToHost, s));
+    GDSJ_HIP(hipStreamSynchronize(s));
+    if (flag) {
+        set_error("fill pass exceeded the output capacity (internal inconsistency)");
+        return GDSJ_EDEVICE;
+    }
+    h->have_table = true;
+    h->st.ms_fill = ms_between(h->ev[4], h->ev[5]);
+    h->st.ms_sort = ms_between(h->ev[5], h->ev[6]);
+    return GDSJ_OK;
+}
+
+int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t dims, int dtype, double epsilon,
+                     const gdsj_params *pin, uint32_t *d_counts, uint64_t *d_offsets, uint32_t *d_ids,
+                     uint64_t ids_cap, uint64_t *total, void *stream) {
+    if (!h || (!d_data && n) || (!d_counts && n) || !total) return GDSJ_EINVAL;
+    if (dtype != GDSJ_DTYPE_F32 && dtype != GDSJ_DTYPE_F64) {
+        set_error("dtype must be GDSJ_DTYPE_F32 or GDSJ_DTYPE_F64");
+        return GDSJ_EINVAL;
+    }
+    GDSJ_HIP(hipSetDevice(h->device));
+    h->have_counts = h->have_table = false;
+    GDSJ_TRY(resolve_params(pin, dims, &h->p));
+    h->stream = stream ? (hipStream_t)stream : h->own;
+    hipStream_t s = h->stream;
+    if (dtype == GDSJ_DTYPE_F32) {
+        float e = (float)epsilon;
+        if ((double)e != epsilon) {
+            set_error("epsilon %.17g is not a float value", epsilon);
+            return GDSJ_EINVAL;
+        }
+        h->thr = (double)gdsj_eps_threshold_f32(e);
+    } else {
+        h->thr = gdsj_eps_threshold_f64(epsilon);
+    }
+    // ILP > 0 splits the sum (kernel.cu:312-342); its fold depends on SHORTCIRCUIT
+    h->mode = h->p.ilp == 0 ? ACC_SEQ : (h->p.shortcircuit ? ACC_ILP_SC : ACC_ILP_NOSC);
+    memset(&h->st, 0, sizeof(h->st));
+    GDSJ_TRY(h->status.ensure(256));
+    GDSJ_HIP(hipEventRecord(h->ev[0], s));
+    uint32_t tq = h->p.tile_queries ? h->p.tile_queries : 64;
+    GDSJ_TRY(grid_build(h->G, d_data, n, dims, dtype, epsilon, h->p.k, h->p.reorder, tq, h->p.queryreorder, s));
+    // ---- counting pass + CSR offsets
+    uint64_t *offs = d_offsets;
+    if (!offs) {
+        GDSJ_TRY(h->offsets.ensure(((size_t)n + 1) * 8));
+        offs = h->offsets.as<uint64_t>();
+    }
+    h->cur_counts = d_counts;
+    h->cur_offsets = offs;
+    JoinArgs a = make_args(h);
+    a.counts = d_counts;
+    GDSJ_TRY(h->scanTmp.ensure(scan_tmp_bytes((uint64_t)n + 1) + 256));
+    GDSJ_HIP(hipEventRecord(h->ev[1], s));
+    GDSJ_TRY(join_pass(h->G, a, h->mode, false, s));
+    GDSJ_HIP(hipEventRecord(h->ev[2], s));
+    GDSJ_TRY(exclusive_scan_u32_u64(d_counts, offs, n, true, h->scanTmp.p, s));
+    GDSJ_HIP(hipEventRecord(h->ev[3], s));
+    uint64_t tot = 0;
+    GDSJ_HIP(hipMemcpyAsync(&tot, offs + n, 8, hipMemcpyDeviceToHost, s));
+    GDSJ_HIP(hipStreamSynchronize(s));
+    h->total = tot;
+    *total = tot;
+    h->have_counts = true;
+    h->st.total_pairs = tot;
+    h->st.candidates = h->G.candidates;
+    h->st.n_nonempty_cells = h->G.ncell;
+    h->st.total_cells = h->G.totalCells;
+    h->st.n_tiles = h->G.ntiles;
+    h->st.n_runs = h->G.nruns;
+    for (uint32_t i = 0; i < dims && i < 64; i++) h->st.perm[i] = h->G.perm[i];
+    h->st.ms_grid = ms_between(h->ev[0], h->ev[1]);
+    h->st.ms_count = ms_between(h->ev[1], h->ev[2]);
+    h->st.ms_scan = ms_between(h->ev[2], h->ev[3]);
+    if (!d_ids) {
+        h->st.ms_total = ms_between(h->ev[0], h->ev[3]);
+        return ids_cap == 0 && tot == 0 ? GDSJ_OK : (tot ? GDSJ_EOVERFLOW : GDSJ_OK);
+    }
+    if (tot > ids_cap) {
+        set_error("result has %llu pairs, output buffer holds %llu", (unsigned long long)tot,
+                  (unsigned long long)ids_cap);
+        h->st.ms_total = ms_between(h->ev[0], h->ev[3]);
+        return GDSJ_EOVERFLOW;
+    }
+    GDSJ_TRY(do_fill(h, d_ids, ids_cap));
+    h->st.ms_total = ms_between(h->ev[0], h->ev[6]);
+    return GDSJ_OK;
+}
+
+int gdsj_fill_device(gdsj_handle *h, uint32_t *d_ids, uint64_t ids_cap) {
+    if (!h) return GDSJ_EINVAL;
+    if (!h->have_counts) {
+        set_error("gdsj_fill_device called without a preceding join");
+        return GDSJ_ESTATE;
+    }
+    if (h->total > ids_cap || (!d_ids && h->total)) {
+        set_error("output buffer too small: need %llu", (unsigned long long)h->total);
+        return GDSJ_EOVERFLOW;
+    }
+    GDSJ_HIP(hipSetDevice(h->device));
+    GDSJ_TRY(do_fill(h, d_ids, ids_cap));
+    h->st.ms_total = h->st.ms_grid + h->st.ms_count + h->st.ms_scan + h->st.ms_fill + h->st.ms_sort;
+    return GDSJ_OK;
+}
+
+int gdsj_join_host(gdsj_handle *h, const void *data, uint32_t n, uint32_t dims, int dtype, double epsilon,
+                   const gdsj_params *p, uint32_t *counts, uint64_t *total) {
+    if (!h || (!data && n) || (!counts && n) || !total) return GDSJ_EINVAL;
+    GDSJ_HIP(hipSetDevice(h->device));
+    size_t es = dtype == GDSJ_DTYPE_F64 ? 8 : 4;
+    size_t bytes = (size_t)n * dims * es;
+    GDSJ_TRY(h->data.ensure(bytes + 256));
+    GDSJ_TRY(h->counts.ensure((size_t)n * 4 + 256));
+    hipStream_t s = h->own;
+    if (bytes) GDSJ_HIP(hipMemcpyAsync(h->data.p, data, bytes, hipMemcpyHostToDevice, s));
+    uint64_t tot = 0;
+    int rc = gdsj_join_device(h, h->data.p, n, dims, dtype, epsilon, p, h->counts.as<uint32_t>(), nullptr,
+                              nullptr, 0, &tot, s);
+    if (rc != GDSJ_OK && rc != GDSJ_EOVERFLOW) return rc;
+    GDSJ_TRY(h->ids.ensure((size_t)tot * 4 + 256));
+    GDSJ_TRY(gdsj_fill_device(h, h->ids.as<uint32_t>(), tot));
+    if (n) GDSJ_HIP(hipMemcpy(counts, h->counts.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    *total = tot;
+    return GDSJ_OK;
+}
+
+int gdsj_result_copy(gdsj_handle *h, uint32_t *ids, uint64_t cap) {
+    if (!h) return GDSJ_EINVAL;
+    if (!h->have_table) {
+        set_error("no table: run gdsj_join_host first");
+        return GDSJ_ESTATE;
+    }
+    if (cap < h->total || (!ids && h->total)) {
+        set_error("buffer holds %llu ids, table has %llu", (unsigned long long)cap, (unsigned long long)h->total);
+        return GDSJ_EOVERFLOW;
+    }
+    GDSJ_HIP(hipSetDevice(h->device));
+    if (h->total) GDSJ_HIP(hipMemcpy(ids, h->ids.p, (size_t)h->total * 4, hipMemcpyDeviceToHost));
+    return GDSJ_OK;
+}
+
+int gdsj_get_stats(gdsj_handle *h, gdsj_stats *out) {
+    if (!h || !out) return GDSJ_EINVAL;
+    *out = h->st;
+    return GDSJ_OK;
+}
+
+// ---------------------------------------------------------------- legacy --
+// The reference keeps the pending table in a process-global pointer
+// (main.cu:359) until copyResultIntoPythonArray copies and frees it
+// (main.cu:507-511).  Here it is a mutex-protected global as well.
+static std::mutex g_legacy_mu;
+static gdsj_params g_legacy_params;
+static bool g_legacy_params_set = false;
+static gdsj_handle *g_legacy_handle[64] = {};
+static uint32_t *g_pending = nullptr;
+static uint64_t g_pending_n = 0;
+static int g_legacy_status = GDSJ_OK;
+
+int gdsj_set_legacy_params(const gdsj_params *p) {
+    std::lock_guard<std::mutex> lk(g_legacy_mu);
+    if (!p) {
+        g_legacy_params_set = false;
+        return GDSJ_OK;
+    }
+    g_legacy_params = *p;
+    g_legacy_params_set = true;
+    return GDSJ_OK;
+}
+
+int gdsj_legacy_status(void) { return g_legacy_status; }
+
+static void legacy_join(const void *data, unsigned int n, double eps, unsigned int ndim, int dtype, int ctx,
+                        unsigned int *counts) {
+    std::lock_guard<std::mutex> lk(g_legacy_mu);
+    if (counts && n) memset(counts, 0, (size_t)n * 4);
+    free(g_pending);
+    g_pending = nullptr;
+    g_pending_n = 0;
+    g_legacy_status = GDSJ_OK;
+    if (ctx < 0 || ctx >= 64) {
+        set_error("gpuContext %d out of range", ctx);
+        g_legacy_status = GDSJ_EINVAL;
+        return;
+    }
+    if (!g_legacy_handle[ctx]) {
+        int rc = gdsj_create(ctx, &g_legacy_handle[ctx]);
+        if (rc) {
+            g_legacy_status = rc;
+            return;
+        }
+    }
+    gdsj_handle *h = g_legacy_handle[ctx];
+    gdsj_params p;
+    if (g_legacy_params_set) p = g_legacy_params; else gdsj_params_default(&p);
+    uint64_t total = 0;
+    int rc = gdsj_join_host(h, data, n, ndim, dtype, eps, &p, counts, &total);
+    if (rc) {
+        if (counts && n) memset(counts, 0, (size_t)n * 4);
+        g_legacy_status = rc;
+        return;
+    }
+    g_pending = (uint32_t *)malloc((size_t)total * 4 + 4);
+    if (!g_pending) {
+        g_legacy_status = GDSJ_EOOM;
+        return;
+    }
+    rc = gdsj_result_copy(h, g_pending, total);
+    if (rc) {
+        free(g_pending);
+        g_pending = nullptr;
+        g_legacy_status = rc;
+        return;
+    }
+    g_pending_n = total;
+}
+
+void GDSJoinPy(float *dataset, unsigned int NUMPOINTS, float epsilon, unsigned int NDIM, int gpuContext,
+               unsigned int *outNumNeighborsWithinEps) {
+    legacy_join(dataset, NUMPOINTS, (double)epsilon, NDIM, GDSJ_DTYPE_F32, gpuContext, outNumNeighborsWithinEps);
+}
+
+void GDSJoinPyDouble(double *dataset, unsigned int NUMPOINTS, double epsilon, unsigned int NDIM, int gpuContext,
+                     unsigned int *outNumNeighborsWithinEps) {
+    legacy_join(dataset, NUMPOINTS, epsilon, NDIM, GDSJ_DTYPE_F64, gpuContext, outNumNeighborsWithinEps);
+}
+
+void copyResultIntoPythonArray(unsigned int *outPythonNeighborTable, unsigned int numResults) {
+    std::lock_guard<std::mutex> lk(g_legacy_mu);
+    if (!g_pending) return;
+    uint64_t m = numResults < g_pending_n ? numResults : g_pending_n;
+    if (outPythonNeighborTable && m) memcpy(outPythonNeighborTable, g_pending, (size_t)m * 4);
+    free(g_pending);
+    g_pending = nullptr;
+    g_pending_n = 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// common.h -- shared device helpers and host plumbing for libgdsjoin.
+// gfx950 (CDNA4) only: wave64, ballot/mbcnt compaction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "gdsjoin.h"
+
+#define GDSJ_WAVE 64
+
+namespace gdsj {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const char *fmt, ...);
+
+#define GDSJ_HIP(call)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            ::gdsj::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,           \
+                              hipGetErrorString(e_));                               \
+            return e_ == hipErrorOutOfMemory ? GDSJ_EOOM : GDSJ_EDEVICE;            \
+        }                                                                           \
+    } while (0)
+
+#define GDSJ_TRY(call)                                                              \
+    do {                                                                            \
+        int rc_ = (call);                                                           \
+        if (rc_ != GDSJ_OK) return rc_;                                             \
+    } while (0)
+
+// ------------------------------------------------------- wave primitives --
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// number of set bits of m in lanes below this lane
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t popc64(uint64_t m) { return (uint32_t)__popcll(m); }
+
+template <typename U>
+__device__ __forceinline__ U uniform(U v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// ------------------------------------------------- exact FP building blocks --
+// Explicit fused multiply-add: the translation units are compiled with
+// -ffp-contract=off so that only these calls fuse.
+__device__ __forceinline__ float fma_rn(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_rn(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// ------------------------------------------------------------ host helpers --
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// device buffer that only grows (scratch reused across joins on one handle)
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap) return GDSJ_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = bytes + bytes / 8 + 256;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error("hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+            return GDSJ_EOOM;
+        }
+        cap = want;
+        return GDSJ_OK;
+    }
+    template <typename U> U *as() const { return (U *)p; }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+}  // namespace gdsj

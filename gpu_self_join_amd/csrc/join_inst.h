@@ -1,0 +1,52 @@
+// join_inst.h -- per-DTYPE dispatch of the join kernel variants.
+// Included by join_f32.hip / join_f64.hip with JT (DTYPE) and JNAME set.
+#include "join_kernels.h"
+
+namespace gdsj {
+
+template <int D, int MODE, int ILPV>
+static int launch_join_t(const Grid &G, const JoinArgs &a, bool fill, hipStream_t s) {
+    const uint32_t waves = 256 / GDSJ_WAVE;
+    const uint32_t blocks = (a.ntiles + waves - 1) / waves;
+    if (blocks == 0) return GDSJ_OK;
+    const JT *xs = (const JT *)a.xs;
+    if (fill)
+        k_join<JT, D, MODE, ILPV, true><<<blocks, 256, 0, s>>>(xs, a.ids, a.tiles, a.runOff, a.runs, a.counts,
+                                                               a.offsets, a.out, a.status, a.ld, a.cap, a.ntiles,
+                                                               a.thr, a.d, a.ilp, a.sc);
+    else
+        k_join<JT, D, MODE, ILPV, false><<<blocks, 256, 0, s>>>(xs, a.ids, a.tiles, a.runOff, a.runs, a.counts,
+                                                                a.offsets, a.out, a.status, a.ld, a.cap,
+                                                                a.ntiles, a.thr, a.d, a.ilp, a.sc);
+    GDSJ_HIP(hipGetLastError());
+    return GDSJ_OK;
+}
+
+#define GDSJ_DIMCASE(DV)                                                              \
+    case DV:                                                                          \
+        if (mode == ACC_SEQ) return launch_join_t<DV, ACC_SEQ, 1>(G, a, fill, s);     \
+        if (mode == ACC_ILP_SC && a.ilp == 8) return launch_join_t<DV, ACC_ILP_SC, 8>(G, a, fill, s); \
+        break;
+
+int JNAME(const Grid &G, const JoinArgs &a, int mode, bool fill, hipStream_t s) {
+    switch (a.d) {
+        JOIN_DIMS
+        default:
+            break;
+    }
+    if (mode == ACC_SEQ) return launch_join_t<0, ACC_SEQ, 0>(G, a, fill, s);
+    if (mode == ACC_ILP_SC) return launch_join_t<0, ACC_ILP_SC, 0>(G, a, fill, s);
+    return launch_join_t<0, ACC_ILP_NOSC, 0>(G, a, fill, s);
+}
+
+bool JNAME_SPECIALISED(uint32_t d, int mode, uint32_t ilp) {
+    switch (d) {
+#define GDSJ_DIMQ(DV) case DV: return mode == ACC_SEQ || (mode == ACC_ILP_SC && ilp == 8);
+        JOIN_DIMQ
+#undef GDSJ_DIMQ
+        default:
+            return false;
+    }
+}
+
+}  // namespace gdsj

@@ -1,0 +1,157 @@
+/*
+ * gdsjoin.h -- C ABI of the MI355X exact Euclidean epsilon-grid self-join
+ * (libgdsjoin.so).  Plain pointers and sizes only.
+ *
+ * Legacy drop-in symbols replace the reference's Python-facing C ABI:
+ *   GDSJoinPy                  main.cu:360   (6-argument layout kept)
+ *   copyResultIntoPythonArray  main.cu:507
+ * The handle API (gdsj_*) replaces the reference's internal host driver for
+ * the same path: ReorderByDimension (main.cu:981), generateNDGridDimensions
+ * (main.cu:891), populateNDGridIndexAndLookupArray (main.cu:690) and
+ * distanceTableNDGridBatches (GPU.cu:270), which the CLI main mode 3 calls
+ * (main.cu:246-254).
+ *
+ * Result contract (identical to the reference on the same inputs): for each
+ * point i, the ids j of every point (i itself included) in a non-empty grid
+ * cell adjacent to i's cell in all k indexed dimensions with
+ * sqrt(acc(i,j)) <= epsilon, where acc is the DTYPE sum of squared
+ * differences accumulated in the reference's FMA order for the chosen ILP
+ * (kernel.cu:307-365).  The table is CSR in point-id order: counts[i] and
+ * the concatenation of every point's neighbor ids.
+ */
+#ifndef GDSJOIN_H
+#define GDSJOIN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference exits the process instead) ---------- */
+#define GDSJ_OK 0
+#define GDSJ_EINVAL (-1)      /* bad argument (dims, k, epsilon <= 0, non-finite data) */
+#define GDSJ_ERANGE (-2)      /* product of cells per dimension overflows uint64 */
+#define GDSJ_EOOM (-3)        /* device or host allocation failed */
+#define GDSJ_EDEVICE (-4)     /* HIP runtime error */
+#define GDSJ_EOVERFLOW (-5)   /* caller's output buffer smaller than the result */
+#define GDSJ_ESTATE (-6)      /* call order violated (e.g. fill before join) */
+#define GDSJ_EUNSUPPORTED (-7)
+
+#define GDSJ_DTYPE_F32 0
+#define GDSJ_DTYPE_F64 1
+
+/* filter modes */
+#define GDSJ_FILTER_PLAIN 0   /* direct differencing only */
+#define GDSJ_FILTER_GRAM 1    /* MFMA Gram-form filter + exact recheck */
+#define GDSJ_FILTER_AUTO 2    /* GRAM when dims >= 16, else PLAIN */
+
+/* result-size estimators (sizing only; never change the result) */
+#define GDSJ_EST_EXACT 0      /* exact counting pass, then fill */
+#define GDSJ_EST_LEARNED 1    /* MLP-predicted per-tile slots; exact pass on overflow */
+#define GDSJ_EST_SAMPLE 2     /* reference sampling estimator (every 66th query) */
+
+/* Runtime form of the reference's compile-time params.h (params.h:17-93).
+ * gdsj_params_default() gives params.h as shipped except dims/k, which
+ * follow the data: k = min(dims, 6). */
+typedef struct gdsj_params {
+    uint32_t k;             /* NUMINDEXEDDIM, 1..12 and <= dims (0 = min(dims,6)) */
+    uint32_t ilp;           /* ILP: 0 = sequential FMA accumulator; else partials */
+    uint32_t shortcircuit;  /* SHORTCIRCUIT (speed only) */
+    uint32_t reorder;       /* REORDER: permute dims by descending variance */
+    uint32_t stamp;         /* STAMP / unicomp (speed only; both directions emitted) */
+    uint32_t queryreorder;  /* QUERYREORDER: heaviest tiles first (speed only) */
+    uint32_t filter;        /* GDSJ_FILTER_* */
+    uint32_t estimator;     /* GDSJ_EST_* */
+    uint32_t sort_neighbors;/* 1: each point's neighbor ids ascending */
+    uint32_t verbose;       /* 1: print progress to stdout */
+    uint32_t tile_queries;  /* queries per work tile, 0 = autotuned */
+    uint32_t reserved[5];
+} gdsj_params;
+
+/* Timing and diagnostics of the last join on a handle. */
+typedef struct gdsj_stats {
+    uint64_t total_pairs;        /* sum of counts, self-pairs included */
+    uint64_t candidates;         /* point comparisons (COUNTMETRICS analogue) */
+    uint64_t n_nonempty_cells;
+    uint64_t total_cells;        /* product of cells per indexed dim */
+    uint64_t n_tiles;
+    uint64_t n_runs;
+    uint64_t filter_rechecks;    /* Gram filter: pairs sent to exact recheck */
+    uint64_t filter_band;        /* Gram filter: rechecks that were not hits */
+    uint64_t est_predicted;      /* estimator: predicted total pairs */
+    uint64_t est_overflow_tiles; /* learned estimator: tiles re-run exactly */
+    double ms_grid;              /* reorder + grid + runs + tiles (device) */
+    double ms_count;             /* counting kernel */
+    double ms_scan;              /* exclusive scan of counts -> CSR offsets */
+    double ms_fill;              /* fill kernel */
+    double ms_sort;              /* per-point row sort (sort_neighbors) */
+    double ms_total;             /* whole join, first to last device event */
+    uint32_t perm[64];           /* dimension order after REORDER (first 64) */
+} gdsj_stats;
+
+typedef struct gdsj_handle gdsj_handle;
+
+/* Fills *p with the defaults (params.h as shipped, k follows the data). */
+int gdsj_params_default(gdsj_params *p);
+
+/* One handle per device; the handle owns its stream and scratch buffers. */
+int gdsj_create(int device, gdsj_handle **out);
+int gdsj_destroy(gdsj_handle *h);
+
+/* Device-resident join.  d_data: n x dims row-major DTYPE on the handle's
+ * device.  d_counts: caller's uint32[n].  d_ids: caller's uint32[ids_cap]
+ * (may be NULL with ids_cap 0 to only count).  d_offsets: optional caller
+ * uint64[n+1] exclusive prefix of counts (NULL = internal).
+ * On return *total holds the number of pairs; GDSJ_EOVERFLOW when
+ * total > ids_cap (counts and offsets are valid, ids untouched), after which
+ * gdsj_fill_device() with a large enough buffer completes the table without
+ * recomputing the grid or the counts.  stream: hipStream_t or NULL for the
+ * handle's own stream; the call returns after the stream has drained. */
+int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t dims,
+                     int dtype, double epsilon, const gdsj_params *p,
+                     uint32_t *d_counts, uint64_t *d_offsets, uint32_t *d_ids,
+                     uint64_t ids_cap, uint64_t *total, void *stream);
+
+/* Completes the table of the last join on h into d_ids (cap >= total). */
+int gdsj_fill_device(gdsj_handle *h, uint32_t *d_ids, uint64_t ids_cap);
+
+/* Host-buffer join (the path GDSJoinPy takes): copies data in, runs the
+ * join, writes counts[n] to host, keeps the table on the handle for
+ * gdsj_result_copy(). */
+int gdsj_join_host(gdsj_handle *h, const void *data, uint32_t n, uint32_t dims,
+                   int dtype, double epsilon, const gdsj_params *p,
+                   uint32_t *counts, uint64_t *total);
+int gdsj_result_copy(gdsj_handle *h, uint32_t *ids, uint64_t cap);
+
+int gdsj_get_stats(gdsj_handle *h, gdsj_stats *out);
+
+/* Largest x with sqrt_rn(x) <= eps, the threshold that replaces the
+ * reference's per-pair square root (kernel.cu:361,364). */
+float gdsj_eps_threshold_f32(float eps);
+double gdsj_eps_threshold_f64(double eps);
+
+/* Thread-local text of the last error. */
+const char *gdsj_last_error(void);
+
+/* ---- legacy drop-in symbols -------------------------------------------- */
+/* main.cu:360.  DTYPE float in this symbol; GDSJoinPyDouble for double.
+ * Dims and k come from the arguments and gdsj_set_legacy_params() instead
+ * of compile-time GPUNUMDIM/NUMINDEXEDDIM.  On error counts are left 0 and
+ * gdsj_last_error() says why (the reference prints and returns). */
+void GDSJoinPy(float *dataset, unsigned int NUMPOINTS, float epsilon, unsigned int NDIM,
+               int gpuContext, unsigned int *outNumNeighborsWithinEps);
+void GDSJoinPyDouble(double *dataset, unsigned int NUMPOINTS, double epsilon,
+                     unsigned int NDIM, int gpuContext, unsigned int *outNumNeighborsWithinEps);
+/* main.cu:507: copies the pending table and frees it (a second call is a no-op). */
+void copyResultIntoPythonArray(unsigned int *outPythonNeighborTable, unsigned int numResults);
+/* Parameters the legacy symbols use (defaults: gdsj_params_default). */
+int gdsj_set_legacy_params(const gdsj_params *p);
+/* Status of the last legacy call (GDSJ_OK or an error code). */
+int gdsj_legacy_status(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GDSJOIN_H */

@@ -1,0 +1,194 @@
+"""Parity of the HIP path (libgdsjoin.so through its C ABI) with the oracle.
+
+Bar: bit-exact counts and identical per-point neighbor sets (rows compared
+after sorting, since the reference's row order follows its atomics).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from gpu_self_join_amd import _lib, gdsjoin
+from oracle import oracle as O
+from tests.helpers import README_COUNT, assert_csr_equal, load_test_dat
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _gpu_join(sj, x, eps, **kw):
+    torch = _torch()
+    t = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    r = sj.join(t, eps, **kw)
+    counts = r.counts.cpu().numpy().view(np.uint32)
+    ids = r.ids.cpu().numpy().view(np.uint32)
+    offs = r.offsets.cpu().numpy()
+    assert offs[0] == 0 and int(offs[-1]) == r.total
+    assert np.array_equal(np.diff(offs), counts.astype(np.int64))
+    return counts, ids, r
+
+
+def _check(sj, x, eps, k, ilp=0, sc=0, reorder=0, **kw):
+    c, ids, r = _gpu_join(sj, x, eps, k=k, ilp=ilp, shortcircuit=sc, reorder=reorder, **kw)
+    ref = O.join(x, eps, k=k, ilp=ilp, shortcircuit=sc, reorder=reorder)
+    print("case n=%d d=%d k=%d ilp=%d sc=%d reorder=%d total=%d" % (x.shape[0], x.shape[1], k, ilp, sc, reorder,
+                                                                    r.total), flush=True)
+    assert_csr_equal(c, ids, ref["counts"], ref["ids"])
+    return r
+
+
+def test_readme_known_answer_gpu(sj):
+    x = load_test_dat(np.float32)
+    r = _check(sj, x, 1.0, k=2, ilp=0, sc=0, reorder=0, queryreorder=0)
+    assert r.total == README_COUNT
+
+
+def test_readme_through_gdsjoin_dropin(gpu):
+    x = load_test_dat(np.float32)
+    p = _lib.default_params(k=2, ilp=0, shortcircuit=0, reorder=0, queryreorder=0)
+    counts, table = gdsjoin(x.ravel(), 1.0, 2, "float", verbose=False, params=p)
+    assert counts.dtype == np.uint32 and table.dtype == np.uint32
+    assert int(counts.sum()) == README_COUNT == len(table)
+    ref = O.join(x, 1.0, k=2)
+    assert_csr_equal(counts, table, ref["counts"], ref["ids"])
+
+
+CASES = [
+    # (n, d, k, dtype, ilp, sc, reorder, neighbors)
+    (3000, 2, 2, np.float32, 0, 0, 0, 30),
+    (3000, 2, 2, np.float32, 8, 1, 0, 30),
+    (4000, 3, 3, np.float32, 8, 1, 1, 40),
+    (5000, 4, 4, np.float32, 0, 1, 0, 20),
+    (6000, 6, 6, np.float32, 8, 1, 1, 64),
+    (4000, 6, 6, np.float32, 8, 0, 1, 30),
+    (3000, 5, 3, np.float32, 0, 0, 0, 25),
+    (3000, 8, 4, np.float32, 8, 1, 1, 30),
+    (2500, 16, 6, np.float32, 8, 1, 1, 30),
+    (2000, 32, 6, np.float32, 8, 1, 1, 32),
+    (1500, 64, 8, np.float32, 8, 1, 1, 20),
+    (2000, 7, 5, np.float32, 3, 1, 1, 25),
+    (2000, 10, 6, np.float32, 0, 0, 1, 25),
+    (3000, 2, 2, np.float64, 0, 0, 0, 30),
+    (3000, 4, 4, np.float64, 8, 1, 1, 30),
+    (2000, 16, 6, np.float64, 8, 1, 1, 30),
+    (1500, 64, 8, np.float64, 8, 1, 1, 50),
+    (1500, 9, 4, np.float64, 8, 0, 0, 20),
+]
+
+
+def _eps_for(x, nb):
+    rng = np.random.default_rng(0)
+    q = x[rng.choice(len(x), 64, replace=False)].astype(np.float64)
+    d = np.sqrt(((q[:, None, :] - x[None, :, :].astype(np.float64)) ** 2).sum(-1))
+    return float(np.asarray(np.sort(d, axis=1)[:, min(nb, len(x) - 1)].mean(), dtype=x.dtype))
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "n%d_d%d_k%d_%s_ilp%d_sc%d_r%d" % (
+    c[0], c[1], c[2], np.dtype(c[3]).name, c[4], c[5], c[6]))
+def test_random_parity(sj, case):
+    n, d, k, dt, ilp, sc, reorder, nb = case
+    rng = np.random.default_rng(n + 31 * d + 7 * k + ilp)
+    ncl = 5
+    centers = rng.random((ncl, d)) * 20
+    x = (centers[rng.integers(0, ncl, n)] + rng.standard_normal((n, d))).astype(dt)
+    eps = _eps_for(x, nb)
+    _check(sj, x, eps, k=k, ilp=ilp, sc=sc, reorder=reorder)
+
+
+def test_dense_cell_many_tiles(sj):
+    # one cell holds far more than 64 queries: tiles split the cell
+    rng = np.random.default_rng(1)
+    x = (rng.random((5000, 2)) * 3).astype(np.float32)
+    _check(sj, x, 1.0, k=2)
+
+
+def test_many_runs_window(sj):
+    # k=6 with uniform data: a cell has far more than 64 neighbor runs
+    rng = np.random.default_rng(2)
+    x = (rng.random((20000, 6)) * 4).astype(np.float32)
+    _check(sj, x, 0.9, k=6, ilp=8, sc=1, reorder=1)
+
+
+def test_edge_single_point(sj):
+    x = np.array([[1.0, 2.0, 3.0]], np.float32)
+    c, ids, r = _gpu_join(sj, x, 0.5, k=3)
+    assert list(c) == [1] and list(ids) == [0]
+
+
+def test_edge_duplicates(sj):
+    x = np.ones((777, 3), np.float64)
+    c, ids, r = _gpu_join(sj, x, 1e-9, k=2)
+    assert np.all(c == 777)
+
+
+def test_edge_isolated(sj):
+    g = np.stack(np.meshgrid(np.arange(40), np.arange(40), indexing="ij"), -1).reshape(-1, 2)
+    x = g.astype(np.float32)
+    c, ids, r = _gpu_join(sj, x, 0.4, k=2)
+    assert np.all(c == 1) and np.array_equal(ids, np.arange(len(x), dtype=np.uint32))
+
+
+def test_edge_all_pairs(sj):
+    rng = np.random.default_rng(4)
+    x = rng.random((700, 3)).astype(np.float32)
+    c, ids, r = _gpu_join(sj, x, 2.0, k=3)
+    assert np.all(c == 700)
+
+
+def test_edge_empty_input(sj):
+    torch = _torch()
+    t = torch.zeros((0, 3), dtype=torch.float32, device="cuda")
+    r = sj.join(t, 1.0, k=3)
+    assert r.total == 0
+
+
+def test_invalid_arguments(sj):
+    torch = _torch()
+    t = torch.zeros((10, 3), dtype=torch.float32, device="cuda")
+    with pytest.raises(_lib.GdsjError):
+        sj.join(t, 0.0, k=3)  # the reference divides by zero here (main.cu:936)
+    with pytest.raises(_lib.GdsjError):
+        sj.join(t, 1.0, k=4)  # k > dims
+    bad = torch.zeros((10, 3), dtype=torch.float32, device="cuda")
+    bad[3, 1] = float("nan")
+    with pytest.raises(_lib.GdsjError):
+        sj.join(bad, 1.0, k=3)
+
+
+def test_sorted_rows_option(sj):
+    rng = np.random.default_rng(6)
+    x = (rng.random((4000, 2)) * 10).astype(np.float32)
+    c, ids, r = _gpu_join(sj, x, 0.8, k=2, sort_neighbors=1)
+    offs = np.concatenate([[0], np.cumsum(c.astype(np.int64))])
+    for i in range(0, len(x), 97):
+        row = ids[offs[i]:offs[i + 1]]
+        assert np.all(row[1:] > row[:-1])
+    ref = O.join(x, 0.8, k=2)
+    assert np.array_equal(ids, ref["ids"])  # sorted rows are exactly the oracle's CSR
+
+
+def test_legacy_symbols_two_call_protocol(gpu):
+    L = _lib.load()
+    rng = np.random.default_rng(8)
+    x = (rng.random((3000, 3)) * 10).astype(np.float32)
+    L.gdsj_set_legacy_params(C.byref(_lib.default_params(k=3, ilp=0, shortcircuit=0, reorder=0)))
+    counts = np.zeros(3000, np.uint32)
+    L.GDSJoinPy(x.ctypes.data, 3000, C.c_float(0.7), 3, 0, counts.ctypes.data)
+    assert L.gdsj_legacy_status() == 0
+    table = np.zeros(int(counts.sum()), np.uint32)
+    L.copyResultIntoPythonArray(table.ctypes.data, len(table))
+    L.copyResultIntoPythonArray(table.ctypes.data, len(table))  # second call is a no-op
+    ref = O.join(x, 0.7, k=3)
+    assert_csr_equal(counts, table, ref["counts"], ref["ids"])
+    L.gdsj_set_legacy_params(None)
+
+
+def test_repeat_join_reuses_handle(sj):
+    rng = np.random.default_rng(10)
+    for n in (5000, 1200, 8000):
+        x = (rng.random((n, 4)) * 10).astype(np.float32)
+        _check(sj, x, 0.9, k=4, ilp=8, sc=1, reorder=1)
