@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--workload", default="gmm6d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=float, default=0.01)
+    ap.add_argument("--param", action="append", default=[],
+                    help="override a gdsj_params field that does not change the result, e.g. queryreorder=0")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the join shards by query cells; "
@@ -77,6 +79,11 @@ def main():
     sj = SelfJoin(0)
     kw = dict(k=params["k"], ilp=params["ilp"], shortcircuit=params["shortcircuit"], reorder=params["reorder"],
               stamp=params["stamp"], queryreorder=params["queryreorder"])
+    for item in args.param:
+        key, val = item.split("=")
+        if key in ("k", "ilp", "reorder"):
+            raise SystemExit("--param %s would change the workload's result contract" % key)
+        kw[key] = int(val)
     counts = torch.empty(n, dtype=torch.int32, device=dev)
     offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
     r = sj.join(pts, eps, counts=counts, offsets=offsets, **kw)
@@ -130,7 +137,7 @@ def main():
         "dtype": "f32" if esz == 4 else "f64",
         "data": "synthetic (seeded %s generator)" % name,
         "config": {"workload": name, "n": n, "dims": d, "eps": eps, "total_pairs": total,
-                   "pairs_per_point": total / n, **params},
+                   "pairs_per_point": total / n, **params, **{k: v for k, v in kw.items() if k not in params}},
         "phase_ms": {k: round(v, 4) for k, v in phase.items()},
         "grid": {"nonempty_cells": s["n_nonempty_cells"], "tiles": s["n_tiles"], "runs": s["n_runs"],
                  "candidates": s["candidates"]},

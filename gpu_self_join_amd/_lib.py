@@ -37,7 +37,8 @@ class GdsjParams(C.Structure):
     _fields_ = [("k", C.c_uint32), ("ilp", C.c_uint32), ("shortcircuit", C.c_uint32),
                 ("reorder", C.c_uint32), ("stamp", C.c_uint32), ("queryreorder", C.c_uint32),
                 ("filter", C.c_uint32), ("estimator", C.c_uint32), ("sort_neighbors", C.c_uint32),
-                ("verbose", C.c_uint32), ("tile_queries", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+                ("verbose", C.c_uint32), ("tile_queries", C.c_uint32), ("schedule", C.c_uint32),
+                ("prune", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class GdsjStats(C.Structure):

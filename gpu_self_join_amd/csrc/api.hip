@@ -42,7 +42,7 @@ struct gdsj_handle {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     Grid G;
-    DevBuf counts, offsets, ids, data, scanTmp, status;
+    DevBuf counts, offsets, ids, data, scanTmp, status, work;
     gdsj_params p{};
     int mode = 0;
     double thr = 0;
@@ -75,6 +75,8 @@ int gdsj_params_default(gdsj_params *p) {
     p->sort_neighbors = 0;
     p->verbose = 0;
     p->tile_queries = 0;
+    p->schedule = 0;
+    p->prune = 0;
     return GDSJ_OK;
 }
 
@@ -132,7 +134,7 @@ int gdsj_destroy(gdsj_handle *h) {
     (void)hipSetDevice(h->device);
     if (h->own) (void)hipStreamSynchronize(h->own);
     h->G.release();
-    DevBuf *b[] = {&h->counts, &h->offsets, &h->ids, &h->data, &h->scanTmp, &h->status};
+    DevBuf *b[] = {&h->counts, &h->offsets, &h->ids, &h->data, &h->scanTmp, &h->status, &h->work};
     for (DevBuf *x : b) x->release();
     for (int i = 0; i < 8; i++)
         if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
@@ -168,13 +170,22 @@ static JoinArgs make_args(gdsj_handle *h) {
     a.ids = h->G.ids.as<uint32_t>();
     a.tiles = h->G.tiles.as<uint4>();
     a.ntiles = h->G.ntiles;
-    a.runOff = h->G.runOff.as<uint32_t>();
-    a.runs = h->G.runs.as<uint2>();
+    a.cellStart = h->G.cellStart.as<uint32_t>();
+    a.cellCoord = h->G.cellCoord.as<uint32_t>();
+    a.hkeys = h->G.hkeys.as<uint64_t>();
+    a.hvals = h->G.hvals.as<uint32_t>();
+    a.ga = h->G.ga;
+    a.ncombo = 1;
+    for (uint32_t j = 1; j < h->G.k; j++) a.ncombo *= 3;
     a.thr = h->thr;
     a.d = h->G.d;
     a.ilp = h->p.ilp;
     a.sc = h->p.shortcircuit;
     a.status = h->status.as<uint32_t>();
+    // auto: round-robin workgroup order (XCD-contiguous ranges unbalance
+    // clustered data); query-box pruning from k >= 4, where it pays
+    a.xcd_map = h->p.schedule == 2 ? 1u : 0u;
+    a.prune = h->p.prune == 1 || (h->p.prune == 0 && h->G.k >= 4) ? 1u : 0u;
     return a;
 }
 
@@ -245,23 +256,29 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     JoinArgs a = make_args(h);
     a.counts = d_counts;
     GDSJ_TRY(h->scanTmp.ensure(scan_tmp_bytes((uint64_t)n + 1) + 256));
+    GDSJ_TRY(h->work.ensure((size_t)h->G.ntiles * 8 + 256));
+    a.tileWork = h->work.as<uint64_t>();
     GDSJ_HIP(hipEventRecord(h->ev[1], s));
     GDSJ_TRY(join_pass(h->G, a, h->mode, false, s));
     GDSJ_HIP(hipEventRecord(h->ev[2], s));
     GDSJ_TRY(exclusive_scan_u32_u64(d_counts, offs, n, true, h->scanTmp.p, s));
     GDSJ_HIP(hipEventRecord(h->ev[3], s));
-    uint64_t tot = 0;
-    GDSJ_HIP(hipMemcpyAsync(&tot, offs + n, 8, hipMemcpyDeviceToHost, s));
+    uint64_t *candSum = h->status.as<uint64_t>() + 4;
+    GDSJ_TRY(sum_u64(a.tileWork, h->G.ntiles, candSum, s));
+    uint64_t hres[2] = {0, 0};
+    GDSJ_HIP(hipMemcpyAsync(&hres[0], offs + n, 8, hipMemcpyDeviceToHost, s));
+    GDSJ_HIP(hipMemcpyAsync(&hres[1], candSum, 8, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipStreamSynchronize(s));
+    uint64_t tot = hres[0];
     h->total = tot;
     *total = tot;
     h->have_counts = true;
     h->st.total_pairs = tot;
-    h->st.candidates = h->G.candidates;
+    h->st.candidates = hres[1];
     h->st.n_nonempty_cells = h->G.ncell;
     h->st.total_cells = h->G.totalCells;
     h->st.n_tiles = h->G.ntiles;
-    h->st.n_runs = h->G.nruns;
+    h->st.n_runs = (uint64_t)h->G.ntiles * a.ncombo;
     for (uint32_t i = 0; i < dims && i < 64; i++) h->st.perm[i] = h->G.perm[i];
     h->st.ms_grid = ms_between(h->ev[0], h->ev[1]);
     h->st.ms_count = ms_between(h->ev[1], h->ev[2]);

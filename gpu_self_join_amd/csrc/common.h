@@ -57,6 +57,17 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  This bijection on [0, nblocks) gives each XCD one contiguous range
+// of logical blocks, so neighboring tiles (which share candidate cells) meet
+// in the same L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
+    constexpr uint32_t NX = 8;
+    const uint32_t per = nblocks / NX, rem = nblocks % NX;
+    const uint32_t x = b % NX, slot = b / NX;
+    return x < rem ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
+}
+
 // ------------------------------------------------- exact FP building blocks --
 // Explicit fused multiply-add: the translation units are compiled with
 // -ffp-contract=off so that only these calls fuse.

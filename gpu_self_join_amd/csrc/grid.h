@@ -1,18 +1,18 @@
 // grid.h -- device epsilon-grid: the MI355X counterpart of the reference's
 // host index build (ReorderByDimension main.cu:981-1075,
 // generateNDGridDimensions main.cu:891-948,
-// populateNDGridIndexAndLookupArray main.cu:690-857) plus the per-cell
-// neighbor-run lists and work tiles the join kernels consume.
+// populateNDGridIndexAndLookupArray main.cu:690-857) plus the cell hash
+// table and work tiles the join kernels consume.
 //
 // HBM layout after build (sorted = points ordered by (cell linear id, id)):
-//   xs      [D][ld] DTYPE   SoA coordinates in sorted order, dims permuted
-//   ids     [n]     uint32  sorted position -> original point id
-//   cellKey [nc]    uint64  sorted unique linear ids (gridCellLookup)
-//   cellStart[nc+1] uint32  sorted-position range of each cell (grid)
-//   runOff  [nc+1]  uint32  per cell: range into runs
-//   runs    [nr]    uint2   {start,end} sorted positions of up to three
-//                           cells consecutive in dimension 0
-//   tiles   [nt]    uint4   {cell, q0, q1, candidates}
+//   xs        [D][ld] DTYPE  SoA coordinates in sorted order, dims permuted
+//   ids       [n]     uint32 sorted position -> original point id
+//   cellKey   [nc]    uint64 sorted unique linear ids (gridCellLookup)
+//   cellStart [nc+1]  uint32 sorted-position range of each cell (grid)
+//   cellCoord [k][nc] uint32 cell coordinates per indexed dim
+//   hkeys/hvals [cap] open-addressing table: linear id + 1 -> cell index
+//                     (replaces the binary searches of kernel.cu:395-398)
+//   tiles     [nt]    uint4  {cell, q0, q1, 0}: up to 64 queries of one cell
 #pragma once
 #include "common.h"
 
@@ -21,11 +21,17 @@ namespace gdsj {
 constexpr int MAXK = 12;
 constexpr int MAXD = 1024;
 
-struct GridDims {
+// what the join kernels need to enumerate adjacent cells
+struct GridArgs {
     uint32_t k;
+    uint32_t ncell;
+    uint32_t hbits;  // log2 of the hash capacity; 0 = direct table indexed by linear id
     uint32_t nCells[MAXK];
-    uint64_t mult[MAXK];  // product of nCells below each dim
-    double minArr[MAXK];  // DTYPE value held exactly in a double
+    uint64_t mult[MAXK];   // product of nCells below each dim
+    double minArr[MAXK];   // grid origin per indexed dim (exact DTYPE values)
+    double eps;            // epsilon (exact DTYPE value)
+    double ulp;            // unit roundoff of DTYPE (2^-24 or 2^-53)
+    double prune_lim;      // eps^2 * (1 + margin): squared gap beyond which no pair can match
 };
 
 struct Grid {
@@ -34,13 +40,13 @@ struct Grid {
     int dtype = 0;
     double eps = 0;
     uint32_t perm[MAXD];
-    GridDims g{};
+    double minArr[MAXK];  // DTYPE value held exactly in a double
+    GridArgs ga{};
     uint64_t totalCells = 0;
-    uint32_t ncell = 0, nruns = 0, ntiles = 0, tileq = 64;
-    uint64_t candidates = 0;  // point comparisons of a full join
-    uint64_t ld = 0;          // leading dimension of xs
+    uint32_t ncell = 0, ntiles = 0, tileq = 64;
+    uint64_t ld = 0;  // leading dimension of xs
     // device buffers
-    DevBuf xs, ids, cellKey, cellStart, runOff, runs, tiles;
+    DevBuf xs, ids, cellKey, cellStart, cellCoord, hkeys, hvals, tiles;
     DevBuf keyA, keyB, valB, tmp, aux, flag, tileKey, tileVal, cellAux, scalars;
     void release();
 };
@@ -49,5 +55,27 @@ struct Grid {
 // reorder: REORDER; tileq: queries per tile (<= 64); order_tiles: QUERYREORDER.
 int grid_build(Grid &G, const void *d_data, uint32_t n, uint32_t d, int dtype, double eps, uint32_t k,
                int reorder, uint32_t tileq, int order_tiles, hipStream_t s);
+
+__device__ __forceinline__ uint32_t cell_hash(uint64_t key, uint32_t bits) {
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - bits));
+}
+
+// cell index of linear id lin, or -1.  Direct table when hbits == 0
+// (hvals indexed by linear id, 0xFFFFFFFF = empty); otherwise linear
+// probing in a table at most half full, so a miss ends at an empty slot.
+__device__ __forceinline__ int32_t cell_find(const uint64_t *__restrict__ hkeys, const uint32_t *__restrict__ hvals,
+                                             uint32_t bits, uint64_t lin) {
+    if (bits == 0) return (int32_t)hvals[lin];
+    const uint32_t mask = (1u << bits) - 1u;
+    uint32_t h = cell_hash(lin, bits);
+    const uint64_t want = lin + 1;
+    for (uint32_t probe = 0; probe <= mask; probe++) {
+        uint64_t k = hkeys[h];
+        if (k == want) return (int32_t)hvals[h];
+        if (k == 0) return -1;
+        h = (h + 1) & mask;
+    }
+    return -1;
+}
 
 }  // namespace gdsj

@@ -48,11 +48,33 @@ __global__ void k_variance(const T *__restrict__ samples, uint32_t m, uint32_t d
     uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= d) return;
     const T *col = samples + (uint64_t)l * m;
+    // loads are batched ahead of the dependent add chain (same order of adds)
+    constexpr uint32_t B = 32;
     T sum = 0;
-    for (uint32_t j = 0; j < m; j++) sum = sum + col[j];
-    T avg = sum / denom;
+    uint32_t j = 0;
+    for (; j + B <= m; j += B) {
+        T v[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++) v[u] = col[j + u];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++) sum = sum + v[u];
+    }
+    for (; j < m; j++) sum = sum + col[j];
+    const T avg = sum / denom;
     sum = 0;
-    for (uint32_t j = 0; j < m; j++) {
+    j = 0;
+    for (; j + B <= m; j += B) {
+        T v[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++) v[u] = col[j + u];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++) {
+            T t = v[u] - avg;
+            T sq = t * t;
+            sum = sum + sq;
+        }
+    }
+    for (; j < m; j++) {
         T t = col[j] - avg;
         T sq = t * t;
         sum = sum + sq;
@@ -108,18 +130,22 @@ __global__ __launch_bounds__(256) void k_minmax(const T *__restrict__ data, uint
 // ----------------------------------------------------------- cell keys --
 template <typename T>
 __global__ void k_cell_keys(const T *__restrict__ data, uint32_t n, uint32_t d, KeyArgs a, T eps,
-                            uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+                            uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                            uint32_t *__restrict__ outside) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint64_t lin = 0;
+    bool bad = false;
     for (uint32_t j = 0; j < a.k; j++) {
         T x = data[(uint64_t)i * d + a.pk[j]];
         T q = (x - (T)a.minArr[j]) / eps;  // IEEE quotient, truncated below
         uint32_t c = (uint32_t)q;
+        bad |= !(q < (T)a.nCells[j]);      // only when eps is below the data's precision
         lin += (uint64_t)c * a.mult[j];
     }
     keys[i] = lin;
     vals[i] = i;
+    if (bad) atomicOr(outside, 1u);
 }
 
 __global__ void k_heads(const uint64_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ flag) {
@@ -160,102 +186,41 @@ __global__ void k_soa(const T *__restrict__ data, uint32_t n, uint32_t d, const 
     if (bad) atomicOr(nonfinite, 1u);
 }
 
-// ------------------------------------------------------ neighbor runs --
-__device__ __forceinline__ uint32_t lower_bound_u64(const uint64_t *__restrict__ a, uint32_t lo, uint32_t hi,
-                                                    uint64_t v) {
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] < v) lo = mid + 1; else hi = mid;
+// ------------------------------------------------- cell hash + coords --
+// Open-addressing table linear id -> cell index, the replacement for the
+// per-candidate-cell binary search of the reference (kernel.cu:388-399).
+__global__ void k_hash_build(const uint64_t *__restrict__ cellKey, uint32_t nc, uint32_t bits,
+                             unsigned long long *__restrict__ hkeys, uint32_t *__restrict__ hvals) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const uint32_t mask = (1u << bits) - 1u;
+    const unsigned long long key = (unsigned long long)cellKey[c] + 1ull;
+    uint32_t h = cell_hash(cellKey[c], bits);
+    for (uint32_t probe = 0; probe <= mask; probe++) {
+        unsigned long long prev = atomicCAS(&hkeys[h], 0ull, key);
+        if (prev == 0ull || prev == key) {
+            hvals[h] = c;
+            return;
+        }
+        h = (h + 1) & mask;
     }
-    return lo;
 }
 
-__device__ __forceinline__ uint32_t upper_bound_u64(const uint64_t *__restrict__ a, uint32_t lo, uint32_t hi,
-                                                    uint64_t v) {
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] <= v) lo = mid + 1; else hi = mid;
-    }
-    return lo;
+__global__ void k_dense_build(const uint64_t *__restrict__ cellKey, uint32_t nc, uint32_t *__restrict__ table) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    table[cellKey[c]] = c;
 }
 
-// Per non-empty cell: every non-empty cell whose coordinates differ by at
-// most one in each indexed dim (the reference's loop nest kernel.cu:292-300
-// with its per-dimension clamps kernel.cu:183-187; its occupancy masks
-// kernel.cu:208-253 only skip coordinates no point occupies, which cannot
-// change the set).  Walks the sorted linear ids from the most significant
-// dim down, pruning empty prefixes; each leaf is a run of up to three cells
-// consecutive in dim 0, emitted as a sorted-position range.
-template <bool FILL>
-__global__ void k_runs(const uint64_t *__restrict__ cellKey, const uint32_t *__restrict__ cellStart, uint32_t nc,
-                       KeyArgs a, const uint32_t *__restrict__ runOff, uint32_t *__restrict__ runCount,
-                       uint2 *__restrict__ runs, uint32_t *__restrict__ cellCand,
-                       unsigned long long *__restrict__ candTotal) {
-    uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t myCand = 0;
-    if (cell < nc) {
-        const uint32_t k = a.k;
-        uint32_t lo[MAXK], hi[MAXK], v[MAXK], A[MAXK], B[MAXK];
-        uint64_t P[MAXK];
-        uint64_t key = cellKey[cell];
-        for (uint32_t j = 0; j < k; j++) {
-            uint32_t c = (uint32_t)(key % a.nCells[j]);
-            key /= a.nCells[j];
-            lo[j] = c == 0 ? 0 : c - 1;
-            uint32_t h = c + 1;
-            hi[j] = (a.nCells[j] - 1 < h) ? a.nCells[j] - 1 : h;
-        }
-        uint32_t nr = 0;
-        uint32_t out = FILL ? runOff[cell] : 0;
-        int j = (int)k - 1;
-        A[j] = 0;
-        B[j] = nc;
-        P[j] = 0;
-        v[j] = lo[j];
-        for (;;) {
-            if (j == 0) {
-                uint32_t x = lower_bound_u64(cellKey, A[0], B[0], P[0] + lo[0]);
-                uint32_t y = upper_bound_u64(cellKey, x, B[0], P[0] + hi[0]);
-                if (x < y) {
-                    uint32_t s0 = cellStart[x], s1 = cellStart[y];
-                    if (FILL) runs[out + nr] = make_uint2(s0, s1);
-                    myCand += s1 - s0;
-                    nr++;
-                }
-                j = 1;
-                if (j > (int)k - 1) break;
-                v[j]++;
-                continue;
-            }
-            if (v[j] > hi[j]) {
-                if (j == (int)k - 1) break;
-                j++;
-                v[j]++;
-                continue;
-            }
-            uint64_t lowId = P[j] + (uint64_t)v[j] * a.mult[j];
-            uint64_t highId = lowId + a.mult[j] - 1;
-            uint32_t x = lower_bound_u64(cellKey, A[j], B[j], lowId);
-            uint32_t y = upper_bound_u64(cellKey, x, B[j], highId);
-            if (x < y) {
-                j--;
-                A[j] = x;
-                B[j] = y;
-                P[j] = lowId;
-                v[j] = lo[j];
-            } else {
-                v[j]++;
-            }
-        }
-        if (!FILL) runCount[cell] = nr;
-        if (FILL) {
-            cellCand[cell] = (uint32_t)myCand;
-            myCand *= (uint64_t)(cellStart[cell + 1] - cellStart[cell]);
-        }
-    }
-    if (FILL) {
-        for (int off = 32; off > 0; off >>= 1) myCand += __shfl_xor(myCand, off, GDSJ_WAVE);
-        if (lane_id() == 0 && myCand) atomicAdd(candTotal, (unsigned long long)myCand);
+// coordinates of each non-empty cell (decomposition of its linear id)
+__global__ void k_cell_coords(const uint64_t *__restrict__ cellKey, uint32_t nc, GridArgs a,
+                              uint32_t *__restrict__ coord) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    uint64_t key = cellKey[c];
+    for (uint32_t j = 0; j < a.k; j++) {
+        coord[(uint64_t)j * nc + c] = (uint32_t)(key % a.nCells[j]);
+        key /= a.nCells[j];
     }
 }
 
@@ -268,21 +233,20 @@ __global__ void k_tiles_count(const uint32_t *__restrict__ cellStart, uint32_t n
     cnt[c] = (m + tileq - 1) / tileq;
 }
 
-__global__ void k_tiles_fill(const uint32_t *__restrict__ cellStart, const uint32_t *__restrict__ cellCand,
-                             uint32_t nc, uint32_t tileq, const uint32_t *__restrict__ tileOff,
-                             uint4 *__restrict__ tiles, uint64_t *__restrict__ tileKey,
-                             uint32_t *__restrict__ tileVal, int order) {
+__global__ void k_tiles_fill(const uint32_t *__restrict__ cellStart, uint32_t nc, uint32_t tileq,
+                             const uint32_t *__restrict__ tileOff, uint4 *__restrict__ tiles,
+                             uint64_t *__restrict__ tileKey, uint32_t *__restrict__ tileVal, int order) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     uint32_t s0 = cellStart[c], s1 = cellStart[c + 1];
     uint32_t t = tileOff[c];
     for (uint32_t q = s0; q < s1; q += tileq, t++) {
         uint32_t q1 = q + tileq < s1 ? q + tileq : s1;
-        tiles[t] = make_uint4(c, q, q1, cellCand[c]);
+        tiles[t] = make_uint4(c, q, q1, 0u);
         if (order) {
-            // heaviest tiles first (QUERYREORDER, GPU.cu:1491-1525, per tile)
-            uint64_t cost = (uint64_t)(q1 - q) * cellCand[c];
-            tileKey[t] = ~cost & 0xFFFFFFFFFFull;
+            // QUERYREORDER (GPU.cu:1491-1525): queries of the most occupied
+            // cells first; stable, so ties keep cell order
+            tileKey[t] = (uint64_t)(~(s1 - s0));
             tileVal[t] = t;
         }
     }
@@ -297,8 +261,8 @@ __global__ void k_tiles_permute(const uint4 *__restrict__ in, const uint32_t *__
 
 // ---------------------------------------------------------------- host --
 void Grid::release() {
-    DevBuf *b[] = {&xs, &ids, &cellKey, &cellStart, &runOff, &runs, &tiles, &keyA, &keyB, &valB,
-                   &tmp, &aux, &flag, &tileKey, &tileVal, &cellAux, &scalars};
+    DevBuf *b[] = {&xs, &ids, &cellKey, &cellStart, &cellCoord, &hkeys, &hvals, &tiles, &keyA, &keyB,
+                   &valB, &tmp, &aux, &flag, &tileKey, &tileVal, &cellAux, &scalars};
     for (DevBuf *p : b) p->release();
 }
 
@@ -322,11 +286,11 @@ static int host_extent(Grid &G, const T *mn, const T *mx) {
             set_error("grid extent out of range in dim %u", j);
             return GDSJ_ERANGE;
         }
-        G.g.nCells[j] = (uint32_t)c;
-        G.g.minArr[j] = (double)lo;
-        G.g.mult[j] = total;
-        uint64_t nt = total * (uint64_t)G.g.nCells[j];
-        if (nt / G.g.nCells[j] != total) {
+        G.ga.nCells[j] = (uint32_t)c;
+        G.minArr[j] = (double)lo;
+        G.ga.mult[j] = total;
+        uint64_t nt = total * (uint64_t)G.ga.nCells[j];
+        if (nt / G.ga.nCells[j] != total) {
             set_error("total grid cells overflow uint64 (dims 0..%u)", j);
             return GDSJ_ERANGE;
         }
@@ -339,6 +303,7 @@ static int host_extent(Grid &G, const T *mn, const T *mx) {
 template <typename T>
 static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int order_tiles) {
     const uint32_t n = G.n, d = G.d, k = G.k;
+    G.ga.k = k;
     // ---- REORDER
     for (uint32_t l = 0; l < d; l++) G.perm[l] = l;
     if (reorder && n > 0) {
@@ -370,7 +335,8 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     a.k = k;
     for (uint32_t j = 0; j < k; j++) a.pk[j] = G.perm[j];
     if (n == 0) {
-        G.ncell = G.nruns = G.ntiles = 0;
+        G.ncell = G.ntiles = 0;
+        G.ga.ncell = 0;
         return GDSJ_OK;
     }
     // ---- extent
@@ -397,9 +363,9 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     }
     GDSJ_TRY(host_extent<T>(G, mn, mx));
     for (uint32_t j = 0; j < k; j++) {
-        a.nCells[j] = G.g.nCells[j];
-        a.mult[j] = G.g.mult[j];
-        a.minArr[j] = G.g.minArr[j];
+        a.nCells[j] = G.ga.nCells[j];
+        a.mult[j] = G.ga.mult[j];
+        a.minArr[j] = G.minArr[j];
     }
     // ---- keys + sort
     GDSJ_TRY(G.keyA.ensure((size_t)n * 8));
@@ -407,9 +373,12 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     GDSJ_TRY(G.ids.ensure((size_t)n * 4 + 256));
     GDSJ_TRY(G.valB.ensure((size_t)n * 4 + 256));
     GDSJ_TRY(G.tmp.ensure(std::max(radix_tmp_bytes(n), scan_tmp_bytes((uint64_t)n + 1)) + 1024));
+    GDSJ_TRY(G.scalars.ensure(256));
     uint64_t *kA = G.keyA.as<uint64_t>(), *kB = G.keyB.as<uint64_t>();
     uint32_t *vA = G.ids.as<uint32_t>(), *vB = G.valB.as<uint32_t>();
-    k_cell_keys<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, a, (T)G.eps, kA, vA);
+    uint32_t *outside = G.scalars.as<uint32_t>() + 1;
+    GDSJ_HIP(hipMemsetAsync(outside, 0, 4, s));
+    k_cell_keys<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, a, (T)G.eps, kA, vA, outside);
     int alt = 0;
     int bits = bit_length(G.totalCells - 1);
     GDSJ_TRY(radix_sort_pairs(kA, vA, kB, vB, n, bits, G.tmp.p, s, &alt));
@@ -421,59 +390,70 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     uint32_t *cidx = flag + n + 1;
     k_heads<<<ceil_div(n, 256), 256, 0, s>>>(skeys, n, flag);
     GDSJ_TRY(exclusive_scan_u32_u32(flag, cidx, n, true, G.tmp.p, s));
-    uint32_t nc = 0;
-    GDSJ_HIP(hipMemcpyAsync(&nc, cidx + n, 4, hipMemcpyDeviceToHost, s));
+    // ---- SoA (sorted order, dims permuted), launched before the cell count sync
+    G.ld = ((uint64_t)n + 63) / 64 * 64;
+    GDSJ_TRY(G.xs.ensure((size_t)G.ld * d * sizeof(T) + 256));
+    uint32_t *dperm = (uint32_t *)((char *)G.aux.p + (((size_t)mmBlocks * k * 2 * sizeof(T) + 255) & ~(size_t)255));
+    GDSJ_HIP(hipMemcpyAsync(dperm, G.perm, d * 4, hipMemcpyHostToDevice, s));
+    uint32_t *nonfinite = G.scalars.as<uint32_t>();
+    uint32_t *idsOut = alt ? vA : vB;  // the value buffer the sort did not end in
+    GDSJ_HIP(hipMemsetAsync(nonfinite, 0, 4, s));
+    k_soa<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, dperm, svals, G.xs.as<T>(), G.ld, idsOut, nonfinite);
+    if (idsOut != G.ids.as<uint32_t>()) std::swap(G.ids, G.valB);
+    uint32_t hdr[3] = {0, 0, 0};
+    GDSJ_HIP(hipMemcpyAsync(&hdr[0], cidx + n, 4, hipMemcpyDeviceToHost, s));
+    GDSJ_HIP(hipMemcpyAsync(&hdr[1], nonfinite, 8, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipStreamSynchronize(s));
+    if (hdr[1]) {
+        set_error("non-finite coordinate in the data");
+        return GDSJ_EINVAL;
+    }
+    if (hdr[2]) {
+        set_error("a cell coordinate falls outside the grid extent: epsilon is below the precision of the data");
+        return GDSJ_ERANGE;
+    }
+    const uint32_t nc = hdr[0];
     G.ncell = nc;
+    G.ga.ncell = nc;
     GDSJ_TRY(G.cellKey.ensure((size_t)nc * 8 + 64));
     GDSJ_TRY(G.cellStart.ensure(((size_t)nc + 1) * 4 + 64));
     k_cells<<<ceil_div(n, 256), 256, 0, s>>>(skeys, flag, cidx, n, G.cellKey.as<uint64_t>(),
                                               G.cellStart.as<uint32_t>());
-    // ---- SoA (sorted order, dims permuted)
-    G.ld = ((uint64_t)n + 63) / 64 * 64;
-    GDSJ_TRY(G.xs.ensure((size_t)G.ld * d * sizeof(T) + 256));
-    // perm to device (in aux, after the min/max partials)
-    uint32_t *dperm = (uint32_t *)((char *)G.aux.p + (((size_t)mmBlocks * k * 2 * sizeof(T) + 255) & ~(size_t)255));
-    GDSJ_HIP(hipMemcpyAsync(dperm, G.perm, d * 4, hipMemcpyHostToDevice, s));
-    GDSJ_TRY(G.scalars.ensure(256));
-    uint32_t *nonfinite = G.scalars.as<uint32_t>();
-    // ids must not alias the sort's value buffer when it is read: write the
-    // final ids into whichever value buffer is not svals
-    uint32_t *idsOut = alt ? vA : vB;
-    GDSJ_HIP(hipMemsetAsync(nonfinite, 0, 4, s));
-    k_soa<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, dperm, svals, G.xs.as<T>(), G.ld, idsOut, nonfinite);
-    if (idsOut != G.ids.as<uint32_t>()) std::swap(G.ids, G.valB);
-    // ---- runs
-    GDSJ_TRY(G.runOff.ensure(((size_t)nc + 1) * 4 + 64));
-    GDSJ_TRY(G.cellAux.ensure(((size_t)nc + 1) * 4 * 4 + 256));
-    uint32_t *runCount = G.cellAux.as<uint32_t>();
-    uint32_t *cellCand = runCount + (nc + 1);
-    uint32_t *tileCnt = cellCand + (nc + 1);
-    uint32_t *tileOff = tileCnt + (nc + 1);
-    unsigned long long *candTotal = (unsigned long long *)(G.scalars.as<uint64_t>() + 1);
-    k_runs<false><<<ceil_div(nc, 64), 64, 0, s>>>(G.cellKey.as<uint64_t>(), G.cellStart.as<uint32_t>(), nc, a,
-                                                  nullptr, runCount, nullptr, nullptr, nullptr);
-    GDSJ_TRY(exclusive_scan_u32_u32(runCount, G.runOff.as<uint32_t>(), nc, true, G.tmp.p, s));
-    uint32_t nr = 0, hbad = 0;
-    GDSJ_HIP(hipMemcpyAsync(&nr, G.runOff.as<uint32_t>() + nc, 4, hipMemcpyDeviceToHost, s));
-    GDSJ_HIP(hipMemcpyAsync(&hbad, nonfinite, 4, hipMemcpyDeviceToHost, s));
-    GDSJ_HIP(hipStreamSynchronize(s));
-    if (hbad) {
-        set_error("non-finite coordinate in the data");
-        return GDSJ_EINVAL;
+    // ---- cell lookup table + coordinates
+    GDSJ_TRY(G.cellCoord.ensure((size_t)nc * k * 4 + 64));
+    if (G.totalCells <= std::max<uint64_t>(4ull * n, 1ull << 20) && G.totalCells <= (1ull << 28)) {
+        // direct table: linear id -> cell index
+        G.ga.hbits = 0;
+        GDSJ_TRY(G.hvals.ensure((size_t)G.totalCells * 4 + 64));
+        GDSJ_HIP(hipMemsetAsync(G.hvals.p, 0xFF, (size_t)G.totalCells * 4, s));
+        k_dense_build<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellKey.as<uint64_t>(), nc, G.hvals.as<uint32_t>());
+        GDSJ_TRY(G.hkeys.ensure(64));
+    } else {
+        uint32_t hb = 6;
+        while ((1ull << hb) < 2ull * nc) hb++;
+        G.ga.hbits = hb;
+        GDSJ_TRY(G.hkeys.ensure(((size_t)1 << hb) * 8));
+        GDSJ_TRY(G.hvals.ensure(((size_t)1 << hb) * 4));
+        GDSJ_HIP(hipMemsetAsync(G.hkeys.p, 0, ((size_t)1 << hb) * 8, s));
+        k_hash_build<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellKey.as<uint64_t>(), nc, hb,
+                                                        G.hkeys.as<unsigned long long>(), G.hvals.as<uint32_t>());
     }
-    G.nruns = nr;
-    GDSJ_TRY(G.runs.ensure((size_t)nr * 8 + 64));
-    GDSJ_HIP(hipMemsetAsync(candTotal, 0, 8, s));
-    k_runs<true><<<ceil_div(nc, 64), 64, 0, s>>>(G.cellKey.as<uint64_t>(), G.cellStart.as<uint32_t>(), nc, a,
-                                                 G.runOff.as<uint32_t>(), nullptr, G.runs.as<uint2>(), cellCand,
-                                                 candTotal);
+    k_cell_coords<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellKey.as<uint64_t>(), nc, G.ga, G.cellCoord.as<uint32_t>());
+    // pruning constants: a neighbor cell whose squared gap to every query of a
+    // tile exceeds eps^2 (1 + 8 (D + 4) u) cannot hold a pair the DTYPE
+    // predicate accepts (the accumulated sum is within (D + 3) u of the exact one)
+    for (uint32_t j = 0; j < k; j++) G.ga.minArr[j] = G.minArr[j];
+    G.ga.eps = G.eps;
+    G.ga.ulp = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
+    G.ga.prune_lim = G.eps * G.eps * (1.0 + 8.0 * (d + 4) * G.ga.ulp);
     // ---- tiles
+    GDSJ_TRY(G.cellAux.ensure(((size_t)nc + 1) * 4 * 2 + 256));
+    uint32_t *tileCnt = G.cellAux.as<uint32_t>();
+    uint32_t *tileOff = tileCnt + (nc + 1);
     k_tiles_count<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellStart.as<uint32_t>(), nc, G.tileq, tileCnt);
     GDSJ_TRY(exclusive_scan_u32_u32(tileCnt, tileOff, nc, true, G.tmp.p, s));
     uint32_t nt = 0;
     GDSJ_HIP(hipMemcpyAsync(&nt, tileOff + nc, 4, hipMemcpyDeviceToHost, s));
-    GDSJ_HIP(hipMemcpyAsync(&G.candidates, candTotal, 8, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipStreamSynchronize(s));
     G.ntiles = nt;
     GDSJ_TRY(G.tiles.ensure((size_t)nt * 16 * 2 + 64));
@@ -484,15 +464,15 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         uint64_t *tk = G.tileKey.as<uint64_t>();
         uint32_t *tv = G.tileVal.as<uint32_t>();
         uint4 *tilesB = tilesA + nt;
-        k_tiles_fill<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellStart.as<uint32_t>(), cellCand, nc, G.tileq,
-                                                        tileOff, tilesB, tk, tv, 1);
+        k_tiles_fill<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellStart.as<uint32_t>(), nc, G.tileq, tileOff, tilesB,
+                                                        tk, tv, 1);
         GDSJ_TRY(G.tmp.ensure(radix_tmp_bytes(nt) + 1024));
         int alt2 = 0;
-        GDSJ_TRY(radix_sort_pairs(tk, tv, tk + nt, tv + nt, nt, 40, G.tmp.p, s, &alt2));
+        GDSJ_TRY(radix_sort_pairs(tk, tv, tk + nt, tv + nt, nt, 32, G.tmp.p, s, &alt2));
         k_tiles_permute<<<ceil_div(nt, 256), 256, 0, s>>>(tilesB, alt2 ? tv + nt : tv, nt, tilesA);
     } else {
-        k_tiles_fill<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellStart.as<uint32_t>(), cellCand, nc, G.tileq,
-                                                        tileOff, tilesA, nullptr, nullptr, 0);
+        k_tiles_fill<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellStart.as<uint32_t>(), nc, G.tileq, tileOff, tilesA,
+                                                        nullptr, nullptr, 0);
     }
     GDSJ_HIP(hipGetLastError());
     return GDSJ_OK;
@@ -515,7 +495,6 @@ int grid_build(Grid &G, const void *d_data, uint32_t n, uint32_t d, int dtype, d
     G.dtype = dtype;
     G.eps = eps;
     G.tileq = tileq;
-    G.candidates = 0;
     if (dtype == GDSJ_DTYPE_F32) return grid_build_t<float>(G, (const float *)d_data, s, reorder, order_tiles);
     if (dtype == GDSJ_DTYPE_F64) return grid_build_t<double>(G, (const double *)d_data, s, reorder, order_tiles);
     set_error("unknown dtype %d", dtype);

@@ -11,13 +11,13 @@ static int launch_join_t(const Grid &G, const JoinArgs &a, bool fill, hipStream_
     if (blocks == 0) return GDSJ_OK;
     const JT *xs = (const JT *)a.xs;
     if (fill)
-        k_join<JT, D, MODE, ILPV, true><<<blocks, 256, 0, s>>>(xs, a.ids, a.tiles, a.runOff, a.runs, a.counts,
-                                                               a.offsets, a.out, a.status, a.ld, a.cap, a.ntiles,
-                                                               a.thr, a.d, a.ilp, a.sc);
+        k_join<JT, D, MODE, ILPV, true><<<blocks, 256, 0, s>>>(
+            xs, a.ids, a.tiles, a.cellStart, a.cellCoord, a.hkeys, a.hvals, a.ga, a.ncombo, a.counts, a.tileWork,
+            a.offsets, a.out, a.status, a.ld, a.cap, a.ntiles, a.thr, a.d, a.ilp, a.sc, a.xcd_map, a.prune);
     else
-        k_join<JT, D, MODE, ILPV, false><<<blocks, 256, 0, s>>>(xs, a.ids, a.tiles, a.runOff, a.runs, a.counts,
-                                                                a.offsets, a.out, a.status, a.ld, a.cap,
-                                                                a.ntiles, a.thr, a.d, a.ilp, a.sc);
+        k_join<JT, D, MODE, ILPV, false><<<blocks, 256, 0, s>>>(
+            xs, a.ids, a.tiles, a.cellStart, a.cellCoord, a.hkeys, a.hvals, a.ga, a.ncombo, a.counts, a.tileWork,
+            a.offsets, a.out, a.status, a.ld, a.cap, a.ntiles, a.thr, a.d, a.ilp, a.sc, a.xcd_map, a.prune);
     GDSJ_HIP(hipGetLastError());
     return GDSJ_OK;
 }

@@ -143,31 +143,125 @@ __device__ __forceinline__ uint32_t cand_pos(uint32_t g, uint32_t nrw, uint32_t 
     return shfl_u32(rstart, idx) + (g - shfl_u32(rexcl, idx));
 }
 
+// Neighbor run of combination t (digits of t in base 3 = offsets -1/0/+1 of
+// dims 1..k-1, dim k-1 most significant, so runs come in ascending linear
+// id): the up to three cells (c0-1, c0, c0+1) of that row, which are
+// consecutive in the sorted cell order.  Coordinates outside
+// [0, nCells-1] are skipped, as the reference's clamps do (kernel.cu:183-187).
+// Cells whose squared gap to the tile's query box exceeds prune_lim are
+// skipped too: no pair with them can pass the predicate.
+__device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK], const double *glo,
+                                           const double *ghi, const GridArgs &ga,
+                                           const uint32_t *__restrict__ cellStart,
+                                           const uint64_t *__restrict__ hkeys,
+                                           const uint32_t *__restrict__ hvals) {
+    uint64_t base = 0;
+    bool ok = true;
+    double gap = 0.0;
+    uint32_t tt = t;
+    for (uint32_t j = 1; j < ga.k; j++) {
+        const uint32_t dlt = tt % 3u;
+        tt /= 3u;
+        const int64_t cj = (int64_t)cc[j] + (int64_t)dlt - 1;
+        ok &= (cj >= 0) && (cj < (int64_t)ga.nCells[j]);
+        gap += dlt == 0 ? glo[j] : (dlt == 2 ? ghi[j] : 0.0);
+        base += (uint64_t)cj * ga.mult[j];
+    }
+    uint2 run = make_uint2(0u, 0u);
+    if (!ok || gap > ga.prune_lim) return run;
+    uint32_t x0 = cc[0] == 0 ? 0u : cc[0] - 1u;
+    uint32_t x1 = cc[0] + 1u < ga.nCells[0] ? cc[0] + 1u : ga.nCells[0] - 1u;
+    if (x0 < cc[0] && gap + glo[0] > ga.prune_lim) x0 = cc[0];
+    if (x1 > cc[0] && gap + ghi[0] > ga.prune_lim) x1 = cc[0];
+    int32_t first = -1, last = -1;
+    for (uint32_t x = x0; x <= x1; x++) {
+        const int32_t ci = cell_find(hkeys, hvals, ga.hbits, base + x);
+        if (ci >= 0) {
+            if (first < 0) first = ci;
+            last = ci;
+        }
+    }
+    if (first >= 0) run = make_uint2(cellStart[first], cellStart[last + 1]);
+    return run;
+}
+
+// Squared gaps from the tile's query bounding box to the cell faces in each
+// indexed dim: glo[j] towards coordinate c_j - 1, ghi[j] towards c_j + 1.
+// Face positions are widened by the classification rounding of
+// (x - min) / eps (a few units of DTYPE roundoff), so the gaps never
+// overstate the distance to any point of the neighbor cell.
+template <typename T>
+__device__ __forceinline__ void tile_gaps(const T *__restrict__ xs, uint64_t ld, uint32_t q0, uint32_t nq,
+                                          const uint32_t (&cc)[MAXK], const GridArgs &ga, double *glo,
+                                          double *ghi) {
+    const uint32_t lane = lane_id();
+    for (uint32_t j = 0; j < ga.k; j++) {
+        double v = lane < nq ? (double)xs[(uint64_t)j * ld + q0 + lane] : 0.0;
+        double mn = lane < nq ? v : 1e300, mx = lane < nq ? v : -1e300;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = fmin(mn, __shfl_xor(mn, off, GDSJ_WAVE));
+            mx = fmax(mx, __shfl_xor(mx, off, GDSJ_WAVE));
+        }
+        const double lo_face = ga.minArr[j] + (double)cc[j] * ga.eps;
+        const double hi_face = lo_face + ga.eps;
+        const double tol = 8.0 * ga.ulp * (fabs(ga.minArr[j]) + fabs(hi_face) + 2.0 * ga.eps);
+        const double gl = fmax(0.0, mn - lo_face - tol);
+        const double gh = fmax(0.0, hi_face - tol - mx);
+        if (lane == 0) {
+            glo[j] = gl * gl;
+            ghi[j] = gh * gh;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <typename T, int D, int MODE, int ILPV, bool FILL>
 __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
-                                              const uint4 *__restrict__ tiles, const uint32_t *__restrict__ runOff,
-                                              const uint2 *__restrict__ runs, uint32_t *__restrict__ counts,
+                                              const uint4 *__restrict__ tiles,
+                                              const uint32_t *__restrict__ cellStart,
+                                              const uint32_t *__restrict__ cellCoord,
+                                              const uint64_t *__restrict__ hkeys,
+                                              const uint32_t *__restrict__ hvals, GridArgs ga, uint32_t ncombo,
+                                              uint32_t *__restrict__ counts, uint64_t *__restrict__ tileWork,
                                               const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
                                               uint32_t *__restrict__ status, uint64_t ld, uint64_t cap,
                                               uint32_t ntiles, double thr_d, uint32_t dd, uint32_t ilp_rt,
-                                              uint32_t sc) {
+                                              uint32_t sc, uint32_t xcd_map, uint32_t prune) {
     constexpr int CMAX = CMaxFor<T, D>::value;
     const uint32_t lane = lane_id();
-    const uint32_t tile = uniform((uint32_t)(blockIdx.x * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
+    const uint32_t lblock = xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t tile = uniform((uint32_t)(lblock * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
     if (tile >= ntiles) return;
     const T thr = (T)thr_d;
     const uint4 tl = tiles[tile];
     const uint32_t cell = uniform(tl.x), q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
-    const uint32_t r0 = uniform(runOff[cell]), r1 = uniform(runOff[cell + 1]);
+    uint32_t cc[MAXK];
+    for (uint32_t j = 0; j < ga.k; j++) cc[j] = uniform(cellCoord[(uint64_t)j * ga.ncell + cell]);
+    // per-wave squared face gaps of this tile (wave-uniform, kept in LDS)
+    __shared__ double s_gap[256 / GDSJ_WAVE][2][MAXK];
+    double *glo = s_gap[threadIdx.x / GDSJ_WAVE][0];
+    double *ghi = s_gap[threadIdx.x / GDSJ_WAVE][1];
+    if (prune) {
+        tile_gaps<T>(xs, ld, q0, nq, cc, ga, glo, ghi);
+    } else {
+        if (lane < MAXK) {
+            glo[lane] = 0.0;
+            ghi[lane] = 0.0;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
     const uint32_t qid_l = lane < nq ? ids[q0 + lane] : 0u;
     uint32_t cnt_l = 0;
     uint64_t cur_l = 0;
+    uint64_t work = 0;
     if (FILL && lane < nq) cur_l = offsets[qid_l];
     bool overflow = false;
 
-    for (uint32_t rw = r0; rw < r1; rw += GDSJ_WAVE) {
-        const uint32_t nrw = uniform(r1 - rw < (uint32_t)GDSJ_WAVE ? r1 - rw : (uint32_t)GDSJ_WAVE);
-        uint2 run = lane < nrw ? runs[rw + lane] : make_uint2(0u, 0u);
+    for (uint32_t cw = 0; cw < ncombo; cw += GDSJ_WAVE) {
+        const uint32_t nrw = uniform(ncombo - cw < (uint32_t)GDSJ_WAVE ? ncombo - cw : (uint32_t)GDSJ_WAVE);
+        uint2 run = lane < nrw ? combo_run(cw + lane, cc, glo, ghi, ga, cellStart, hkeys, hvals)
+                               : make_uint2(0u, 0u);
         uint32_t len = run.y - run.x;
         uint32_t incl = len;
 #pragma unroll
@@ -176,9 +270,12 @@ __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const ui
             if (lane >= (uint32_t)off) incl += o;
         }
         const uint32_t total = uniform(shfl_u32(incl, nrw - 1));
+        if (total == 0) continue;
+        work += total;
         const uint32_t rexcl = incl - len;
         const uint32_t rstart = run.x;
-        const uint32_t first = uniform(run.x);
+        // a valid position for lanes past the end: first candidate of the window
+        const uint32_t fpos = uniform(cand_pos(0u, nrw, rstart, rexcl));
 
         for (uint32_t cb = 0; cb < total; cb += GDSJ_WAVE * CMAX) {
             const uint32_t nch = uniform((total - cb + GDSJ_WAVE - 1) / GDSJ_WAVE < (uint32_t)CMAX
@@ -193,7 +290,7 @@ __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const ui
                 const uint32_t g = cb + ch * GDSJ_WAVE + lane;
                 valid[ch] = g < total;
                 uint32_t sp = cand_pos(valid[ch] ? g : 0u, nrw, rstart, rexcl);
-                sp = valid[ch] ? sp : first;
+                sp = valid[ch] ? sp : fpos;
                 cpos[ch] = sp;
                 if (D > 0 && ch < (int)nch) {
 #pragma unroll
@@ -249,6 +346,7 @@ __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const ui
         }
     }
     if (!FILL && lane < nq) counts[qid_l] = cnt_l;
+    if (!FILL && tileWork && lane == 0) tileWork[tile] = work * nq;
     if (FILL && overflow) atomicOr(status, 1u);
 }
 

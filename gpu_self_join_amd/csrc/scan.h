@@ -49,4 +49,7 @@ int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *out, uint64_t n, bool w
 int exclusive_scan_u32_u32(const uint32_t *in, uint32_t *out, uint64_t n, bool write_total,
                            void *tmp, hipStream_t s);
 
+// *out = sum of in[0..n) (uint64), on stream s.
+int sum_u64(const uint64_t *in, uint64_t n, uint64_t *out, hipStream_t s);
+
 }  // namespace gdsj

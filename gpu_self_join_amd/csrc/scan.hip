@@ -70,6 +70,26 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const In *__restrict__
     if (write_total && blockIdx.x == 0 && threadIdx.x == 0) out[n] = (Out)partial[nb];
 }
 
+__global__ __launch_bounds__(SCAN_BLOCK) void k_sum_u64(const uint64_t *__restrict__ in, uint64_t n,
+                                                         unsigned long long *__restrict__ out) {
+    uint64_t v = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * SCAN_BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * SCAN_BLOCK)
+        v += in[i];
+    uint64_t tot;
+    (void)block_exclusive_scan<uint64_t>(v, &tot);
+    if (threadIdx.x == 0 && tot) atomicAdd(out, (unsigned long long)tot);
+}
+
+int sum_u64(const uint64_t *in, uint64_t n, uint64_t *out, hipStream_t s) {
+    GDSJ_HIP(hipMemsetAsync(out, 0, 8, s));
+    if (n == 0) return GDSJ_OK;
+    uint32_t blocks = (uint32_t)((n + SCAN_BLOCK * 16 - 1) / (SCAN_BLOCK * 16));
+    if (blocks > 1024) blocks = 1024;
+    k_sum_u64<<<blocks, SCAN_BLOCK, 0, s>>>(in, n, (unsigned long long *)out);
+    GDSJ_HIP(hipGetLastError());
+    return GDSJ_OK;
+}
+
 size_t scan_tmp_bytes(uint64_t n) {
     uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     return (nb + 2) * sizeof(uint64_t);

@@ -66,7 +66,10 @@ typedef struct gdsj_params {
     uint32_t sort_neighbors;/* 1: each point's neighbor ids ascending */
     uint32_t verbose;       /* 1: print progress to stdout */
     uint32_t tile_queries;  /* queries per work tile, 0 = autotuned */
-    uint32_t reserved[5];
+    uint32_t schedule;      /* tile->workgroup order: 0 auto, 1 round-robin, 2 XCD-contiguous */
+    uint32_t prune;         /* skip neighbor cells farther than eps from a tile's query box:
+                               0 auto, 1 on, 2 off (speed only) */
+    uint32_t reserved[3];
 } gdsj_params;
 
 /* Timing and diagnostics of the last join on a handle. */

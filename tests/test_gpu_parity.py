@@ -192,3 +192,20 @@ def test_repeat_join_reuses_handle(sj):
     for n in (5000, 1200, 8000):
         x = (rng.random((n, 4)) * 10).astype(np.float32)
         _check(sj, x, 0.9, k=4, ilp=8, sc=1, reorder=1)
+
+
+@pytest.mark.parametrize("d,k,dt,jitter", [(2, 2, np.float32, 0.0), (3, 3, np.float64, 0.0), (6, 6, np.float32, 0.0),
+                                           (6, 6, np.float32, 1e-6), (4, 2, np.float64, 1e-12),
+                                           (8, 8, np.float32, 0.0)])
+def test_lattice_on_cell_faces(sj, d, k, dt, jitter):
+    # points on a lattice of spacing eps: many pairs at distance exactly eps and
+    # every point on a cell face, where classification rounding decides the cell
+    rng = np.random.default_rng(d * 10 + k)
+    side = {2: 60, 3: 16, 4: 7, 6: 5, 8: 3}[d]
+    g = np.stack(np.meshgrid(*[np.arange(side)] * d, indexing="ij"), -1).reshape(-1, d)
+    eps = 0.37
+    x = (g * eps + 3.1).astype(np.float64)
+    x += jitter * rng.standard_normal(x.shape)
+    x = x.astype(dt)
+    for ilp, sc, reorder in [(0, 0, 0), (8, 1, 1)]:
+        _check(sj, x, float(np.asarray(eps, dtype=dt)), k=k, ilp=ilp, sc=sc, reorder=reorder)
