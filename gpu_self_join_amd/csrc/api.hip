@@ -186,6 +186,11 @@ static JoinArgs make_args(gdsj_handle *h) {
     // clustered data); query-box pruning from k >= 4, where it pays
     a.xcd_map = h->p.schedule == 2 ? 1u : 0u;
     a.prune = h->p.prune == 1 || (h->p.prune == 0 && h->G.k >= 4) ? 1u : 0u;
+    if (h->G.haveRuns) {
+        a.runs = h->G.runs.as<uint2>();
+        a.nruns = h->G.nruns.as<uint32_t>();
+        a.runStride = h->G.runStride;
+    }
     return a;
 }
 
@@ -197,7 +202,7 @@ static int do_fill(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
     a.cap = cap;
     GDSJ_HIP(hipMemsetAsync(h->status.p, 0, 4, s));
     GDSJ_HIP(hipEventRecord(h->ev[4], s));
-    GDSJ_TRY(join_pass(h->G, a, h->mode, true, s));
+    GDSJ_TRY(join_pass(h->G, a, h->mode, PASS_FILL, s));
     GDSJ_HIP(hipEventRecord(h->ev[5], s));
     if (h->p.sort_neighbors)
         GDSJ_TRY(sort_rows(h->cur_offsets, h->G.n, d_ids, s));
@@ -245,6 +250,12 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     GDSJ_HIP(hipEventRecord(h->ev[0], s));
     uint32_t tq = h->p.tile_queries ? h->p.tile_queries : 64;
     GDSJ_TRY(grid_build(h->G, d_data, n, dims, dtype, epsilon, h->p.k, h->p.reorder, tq, h->p.queryreorder, s));
+    {
+        bool built = false;
+        const bool prune = h->p.prune == 1 || (h->p.prune == 0 && h->G.k >= 4);
+        GDSJ_TRY(build_runs(h->G, s, prune, &built));
+        h->G.haveRuns = built;
+    }
     // ---- counting pass + CSR offsets
     uint64_t *offs = d_offsets;
     if (!offs) {
@@ -259,7 +270,7 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     GDSJ_TRY(h->work.ensure((size_t)h->G.ntiles * 8 + 256));
     a.tileWork = h->work.as<uint64_t>();
     GDSJ_HIP(hipEventRecord(h->ev[1], s));
-    GDSJ_TRY(join_pass(h->G, a, h->mode, false, s));
+    GDSJ_TRY(join_pass(h->G, a, h->mode, PASS_COUNT, s));
     GDSJ_HIP(hipEventRecord(h->ev[2], s));
     GDSJ_TRY(exclusive_scan_u32_u64(d_counts, offs, n, true, h->scanTmp.p, s));
     GDSJ_HIP(hipEventRecord(h->ev[3], s));

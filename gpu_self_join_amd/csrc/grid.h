@@ -47,6 +47,9 @@ struct Grid {
     uint64_t ld = 0;  // leading dimension of xs
     // device buffers
     DevBuf xs, ids, cellKey, cellStart, cellCoord, hkeys, hvals, tiles;
+    DevBuf runs, nruns, cellCand;  // per-cell neighbor runs (build_runs)
+    uint64_t runStride = 0;
+    bool haveRuns = false;
     DevBuf keyA, keyB, valB, tmp, aux, flag, tileKey, tileVal, cellAux, scalars;
     void release();
 };

@@ -12,34 +12,60 @@ enum AccMode : int {
     ACC_ILP_NOSC = 2,  // ILP > 0, SHORTCIRCUIT == 0: ILP partials, fold at end
 };
 
+enum JoinPass : int { PASS_COUNT = 0, PASS_FILL = 1, PASS_SLOT = 2 };
+
+// Everything the join kernels read besides the coordinates and ids
+// (passed by value as one kernel argument).
 struct JoinArgs {
     const void *xs;
     uint64_t ld;
     const uint32_t *ids;
     const uint4 *tiles;
+    const uint32_t *tileList;  // nullable: process tiles[tileList[t]] for t < ntiles
     uint32_t ntiles;
     const uint32_t *cellStart;
     const uint32_t *cellCoord;
     const uint64_t *hkeys;
     const uint32_t *hvals;
+    const uint2 *runs;         // nullable: precomputed neighbor runs per cell
+    const uint32_t *nruns;     //   number of runs of each cell
+    uint64_t runStride;        //   runs of cell c start at c * runStride
     GridArgs ga;
     uint32_t ncombo;   // 3^(k-1) neighbor combinations of dims 1..k-1
     double thr;        // largest DTYPE value x with sqrt(x) <= eps (exact in double)
     uint32_t d;        // dims
     uint32_t ilp;      // ILP
     uint32_t sc;       // SHORTCIRCUIT early exit
-    uint32_t *counts;  // count pass: counts[original id]
-    uint64_t *tileWork;  // count pass: queries x candidates per tile (nullable)
-    const uint64_t *offsets;  // fill pass: CSR offsets by original id
-    uint32_t *out;     // fill pass: neighbor ids
-    uint64_t cap;      // capacity of out
-    uint32_t *status;  // [0] set when a write would exceed cap
     uint32_t xcd_map;  // 1: XCD-contiguous block->tile mapping
-    uint32_t prune;    // 1: query-box pruning of neighbor cells
+    uint32_t prune;    // 1: query-box pruning of neighbor cells (in-kernel enumeration)
+    // count / slot pass
+    uint32_t *counts;    // counts[original id]
+    uint64_t *tileWork;  // queries x candidates per tile (nullable)
+    // fill pass
+    const uint64_t *offsets;  // CSR offsets by original id
+    uint32_t *out;            // neighbor ids
+    uint64_t cap;             // capacity of out
+    // slot pass: hits go to a per-tile slot sized by the estimator
+    const uint64_t *slotOff;  // slot start per tile
+    const uint32_t *slotCap;  // slot capacity per tile
+    uint32_t *slotUsed;       // hits per tile (may exceed slotCap: overflow)
+    uint32_t *slotIds;
+    uint8_t *slotQ;           // query index inside the tile of each slot entry
+    uint32_t *ovfList;        // tiles whose slot overflowed
+    uint32_t *ovfCount;
+    uint32_t *status;  // [0] set when a fill write would exceed cap
 };
 
-// Launches the count (fill=false) or fill (fill=true) pass on stream s.
-int join_pass(const Grid &G, const JoinArgs &a, int mode, bool fill, hipStream_t s);
+// Launches one pass (JoinPass) of the join over a.ntiles tiles on stream s.
+int join_pass(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t s);
+
+// Builds the compacted neighbor-run list of every cell (runs, nruns,
+// cellCand = candidates per query of the cell) when it fits the memory cap;
+// returns false (and leaves the lists unset) otherwise.
+int build_runs(Grid &G, hipStream_t s, bool prune, bool *built);
+
+// Moves every non-overflowed tile's slot entries into CSR rows.
+int compact_slots(const JoinArgs &a, hipStream_t s);
 
 // Name of the kernel variant join_pass picks (for profiles and tests).
 const char *join_variant(int dtype, uint32_t d, int mode, uint32_t ilp);

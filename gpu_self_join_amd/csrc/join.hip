@@ -3,14 +3,14 @@
 
 namespace gdsj {
 
-int join_pass_f32(const Grid &G, const JoinArgs &a, int mode, bool fill, hipStream_t s);
-int join_pass_f64(const Grid &G, const JoinArgs &a, int mode, bool fill, hipStream_t s);
+int join_pass_f32(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t s);
+int join_pass_f64(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t s);
 bool join_specialised_f32(uint32_t d, int mode, uint32_t ilp);
 bool join_specialised_f64(uint32_t d, int mode, uint32_t ilp);
 
-int join_pass(const Grid &G, const JoinArgs &a, int mode, bool fill, hipStream_t s) {
-    if (G.dtype == GDSJ_DTYPE_F32) return join_pass_f32(G, a, mode, fill, s);
-    return join_pass_f64(G, a, mode, fill, s);
+int join_pass(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t s) {
+    if (G.dtype == GDSJ_DTYPE_F32) return join_pass_f32(G, a, mode, pass, s);
+    return join_pass_f64(G, a, mode, pass, s);
 }
 
 const char *join_variant(int dtype, uint32_t d, int mode, uint32_t ilp) {

@@ -5,38 +5,36 @@
 namespace gdsj {
 
 template <int D, int MODE, int ILPV>
-static int launch_join_t(const Grid &G, const JoinArgs &a, bool fill, hipStream_t s) {
+static int launch_join_t(const Grid &G, const JoinArgs &a, int pass, hipStream_t s) {
     const uint32_t waves = 256 / GDSJ_WAVE;
     const uint32_t blocks = (a.ntiles + waves - 1) / waves;
     if (blocks == 0) return GDSJ_OK;
     const JT *xs = (const JT *)a.xs;
-    if (fill)
-        k_join<JT, D, MODE, ILPV, true><<<blocks, 256, 0, s>>>(
-            xs, a.ids, a.tiles, a.cellStart, a.cellCoord, a.hkeys, a.hvals, a.ga, a.ncombo, a.counts, a.tileWork,
-            a.offsets, a.out, a.status, a.ld, a.cap, a.ntiles, a.thr, a.d, a.ilp, a.sc, a.xcd_map, a.prune);
+    if (pass == PASS_FILL)
+        k_join<JT, D, MODE, ILPV, PASS_FILL><<<blocks, 256, 0, s>>>(xs, a.ids, a);
+    else if (pass == PASS_SLOT)
+        k_join<JT, D, MODE, ILPV, PASS_SLOT><<<blocks, 256, 0, s>>>(xs, a.ids, a);
     else
-        k_join<JT, D, MODE, ILPV, false><<<blocks, 256, 0, s>>>(
-            xs, a.ids, a.tiles, a.cellStart, a.cellCoord, a.hkeys, a.hvals, a.ga, a.ncombo, a.counts, a.tileWork,
-            a.offsets, a.out, a.status, a.ld, a.cap, a.ntiles, a.thr, a.d, a.ilp, a.sc, a.xcd_map, a.prune);
+        k_join<JT, D, MODE, ILPV, PASS_COUNT><<<blocks, 256, 0, s>>>(xs, a.ids, a);
     GDSJ_HIP(hipGetLastError());
     return GDSJ_OK;
 }
 
 #define GDSJ_DIMCASE(DV)                                                              \
     case DV:                                                                          \
-        if (mode == ACC_SEQ) return launch_join_t<DV, ACC_SEQ, 1>(G, a, fill, s);     \
-        if (mode == ACC_ILP_SC && a.ilp == 8) return launch_join_t<DV, ACC_ILP_SC, 8>(G, a, fill, s); \
+        if (mode == ACC_SEQ) return launch_join_t<DV, ACC_SEQ, 1>(G, a, pass, s);     \
+        if (mode == ACC_ILP_SC && a.ilp == 8) return launch_join_t<DV, ACC_ILP_SC, 8>(G, a, pass, s); \
         break;
 
-int JNAME(const Grid &G, const JoinArgs &a, int mode, bool fill, hipStream_t s) {
+int JNAME(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t s) {
     switch (a.d) {
         JOIN_DIMS
         default:
             break;
     }
-    if (mode == ACC_SEQ) return launch_join_t<0, ACC_SEQ, 0>(G, a, fill, s);
-    if (mode == ACC_ILP_SC) return launch_join_t<0, ACC_ILP_SC, 0>(G, a, fill, s);
-    return launch_join_t<0, ACC_ILP_NOSC, 0>(G, a, fill, s);
+    if (mode == ACC_SEQ) return launch_join_t<0, ACC_SEQ, 0>(G, a, pass, s);
+    if (mode == ACC_ILP_SC) return launch_join_t<0, ACC_ILP_SC, 0>(G, a, pass, s);
+    return launch_join_t<0, ACC_ILP_NOSC, 0>(G, a, pass, s);
 }
 
 bool JNAME_SPECIALISED(uint32_t d, int mode, uint32_t ilp) {

@@ -15,6 +15,7 @@
 // largest DTYPE value whose correctly rounded square root is <= eps.
 #pragma once
 #include "join.h"
+#include "neighbors.h"
 
 namespace gdsj {
 
@@ -128,140 +129,61 @@ __device__ __forceinline__ T dist_mem(const T *__restrict__ xs, uint64_t ld, uin
     return acc;
 }
 
-__device__ __forceinline__ uint32_t shfl_u32(uint32_t v, uint32_t src) { return __shfl(v, (int)src, GDSJ_WAVE); }
-
-// lane -> candidate sorted position for chunk-relative index g within a
-// window of up to 64 runs (run i held by lane i: start, exclusive prefix)
-__device__ __forceinline__ uint32_t cand_pos(uint32_t g, uint32_t nrw, uint32_t rstart, uint32_t rexcl) {
-    uint32_t idx = 0;
-#pragma unroll
-    for (uint32_t step = 32; step > 0; step >>= 1) {
-        uint32_t cnd = idx + step;
-        uint32_t e = shfl_u32(rexcl, cnd < 64 ? cnd : 63);
-        if (cnd < nrw && e <= g) idx = cnd;
-    }
-    return shfl_u32(rstart, idx) + (g - shfl_u32(rexcl, idx));
-}
-
-// Neighbor run of combination t (digits of t in base 3 = offsets -1/0/+1 of
-// dims 1..k-1, dim k-1 most significant, so runs come in ascending linear
-// id): the up to three cells (c0-1, c0, c0+1) of that row, which are
-// consecutive in the sorted cell order.  Coordinates outside
-// [0, nCells-1] are skipped, as the reference's clamps do (kernel.cu:183-187).
-// Cells whose squared gap to the tile's query box exceeds prune_lim are
-// skipped too: no pair with them can pass the predicate.
-__device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK], const double *glo,
-                                           const double *ghi, const GridArgs &ga,
-                                           const uint32_t *__restrict__ cellStart,
-                                           const uint64_t *__restrict__ hkeys,
-                                           const uint32_t *__restrict__ hvals) {
-    uint64_t base = 0;
-    bool ok = true;
-    double gap = 0.0;
-    uint32_t tt = t;
-    for (uint32_t j = 1; j < ga.k; j++) {
-        const uint32_t dlt = tt % 3u;
-        tt /= 3u;
-        const int64_t cj = (int64_t)cc[j] + (int64_t)dlt - 1;
-        ok &= (cj >= 0) && (cj < (int64_t)ga.nCells[j]);
-        gap += dlt == 0 ? glo[j] : (dlt == 2 ? ghi[j] : 0.0);
-        base += (uint64_t)cj * ga.mult[j];
-    }
-    uint2 run = make_uint2(0u, 0u);
-    if (!ok || gap > ga.prune_lim) return run;
-    uint32_t x0 = cc[0] == 0 ? 0u : cc[0] - 1u;
-    uint32_t x1 = cc[0] + 1u < ga.nCells[0] ? cc[0] + 1u : ga.nCells[0] - 1u;
-    if (x0 < cc[0] && gap + glo[0] > ga.prune_lim) x0 = cc[0];
-    if (x1 > cc[0] && gap + ghi[0] > ga.prune_lim) x1 = cc[0];
-    int32_t first = -1, last = -1;
-    for (uint32_t x = x0; x <= x1; x++) {
-        const int32_t ci = cell_find(hkeys, hvals, ga.hbits, base + x);
-        if (ci >= 0) {
-            if (first < 0) first = ci;
-            last = ci;
-        }
-    }
-    if (first >= 0) run = make_uint2(cellStart[first], cellStart[last + 1]);
-    return run;
-}
-
-// Squared gaps from the tile's query bounding box to the cell faces in each
-// indexed dim: glo[j] towards coordinate c_j - 1, ghi[j] towards c_j + 1.
-// Face positions are widened by the classification rounding of
-// (x - min) / eps (a few units of DTYPE roundoff), so the gaps never
-// overstate the distance to any point of the neighbor cell.
-template <typename T>
-__device__ __forceinline__ void tile_gaps(const T *__restrict__ xs, uint64_t ld, uint32_t q0, uint32_t nq,
-                                          const uint32_t (&cc)[MAXK], const GridArgs &ga, double *glo,
-                                          double *ghi) {
-    const uint32_t lane = lane_id();
-    for (uint32_t j = 0; j < ga.k; j++) {
-        double v = lane < nq ? (double)xs[(uint64_t)j * ld + q0 + lane] : 0.0;
-        double mn = lane < nq ? v : 1e300, mx = lane < nq ? v : -1e300;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            mn = fmin(mn, __shfl_xor(mn, off, GDSJ_WAVE));
-            mx = fmax(mx, __shfl_xor(mx, off, GDSJ_WAVE));
-        }
-        const double lo_face = ga.minArr[j] + (double)cc[j] * ga.eps;
-        const double hi_face = lo_face + ga.eps;
-        const double tol = 8.0 * ga.ulp * (fabs(ga.minArr[j]) + fabs(hi_face) + 2.0 * ga.eps);
-        const double gl = fmax(0.0, mn - lo_face - tol);
-        const double gh = fmax(0.0, hi_face - tol - mx);
-        if (lane == 0) {
-            glo[j] = gl * gl;
-            ghi[j] = gh * gh;
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-}
-
-template <typename T, int D, int MODE, int ILPV, bool FILL>
+template <typename T, int D, int MODE, int ILPV, int PASS>
 __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
-                                              const uint4 *__restrict__ tiles,
-                                              const uint32_t *__restrict__ cellStart,
-                                              const uint32_t *__restrict__ cellCoord,
-                                              const uint64_t *__restrict__ hkeys,
-                                              const uint32_t *__restrict__ hvals, GridArgs ga, uint32_t ncombo,
-                                              uint32_t *__restrict__ counts, uint64_t *__restrict__ tileWork,
-                                              const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
-                                              uint32_t *__restrict__ status, uint64_t ld, uint64_t cap,
-                                              uint32_t ntiles, double thr_d, uint32_t dd, uint32_t ilp_rt,
-                                              uint32_t sc, uint32_t xcd_map, uint32_t prune) {
+                                              const JoinArgs a) {
     constexpr int CMAX = CMaxFor<T, D>::value;
+    constexpr bool NEED_IDS = PASS != PASS_COUNT;
     const uint32_t lane = lane_id();
-    const uint32_t lblock = xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const uint32_t tile = uniform((uint32_t)(lblock * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
-    if (tile >= ntiles) return;
-    const T thr = (T)thr_d;
-    const uint4 tl = tiles[tile];
+    const uint32_t lblock = a.xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t tslot = uniform((uint32_t)(lblock * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
+    if (tslot >= a.ntiles) return;
+    const uint32_t tile = a.tileList ? uniform(a.tileList[tslot]) : tslot;
+    const uint64_t ld = a.ld;
+    const T thr = (T)a.thr;
+    const uint4 tl = a.tiles[tile];
     const uint32_t cell = uniform(tl.x), q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
+    const GridArgs &ga = a.ga;
+    const bool pre = a.runs != nullptr;
     uint32_t cc[MAXK];
-    for (uint32_t j = 0; j < ga.k; j++) cc[j] = uniform(cellCoord[(uint64_t)j * ga.ncell + cell]);
-    // per-wave squared face gaps of this tile (wave-uniform, kept in LDS)
     __shared__ double s_gap[256 / GDSJ_WAVE][2][MAXK];
     double *glo = s_gap[threadIdx.x / GDSJ_WAVE][0];
     double *ghi = s_gap[threadIdx.x / GDSJ_WAVE][1];
-    if (prune) {
-        tile_gaps<T>(xs, ld, q0, nq, cc, ga, glo, ghi);
+    uint32_t nwin;
+    if (pre) {
+        nwin = uniform(a.nruns[cell]);
     } else {
-        if (lane < MAXK) {
-            glo[lane] = 0.0;
-            ghi[lane] = 0.0;
+        nwin = a.ncombo;
+        for (uint32_t j = 0; j < ga.k; j++) cc[j] = uniform(a.cellCoord[(uint64_t)j * ga.ncell + cell]);
+        if (a.prune) {
+            tile_gaps<T>(xs, ld, q0, nq, cc, ga, glo, ghi);
+        } else {
+            if (lane < MAXK) {
+                glo[lane] = 0.0;
+                ghi[lane] = 0.0;
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
     }
     const uint32_t qid_l = lane < nq ? ids[q0 + lane] : 0u;
     uint32_t cnt_l = 0;
     uint64_t cur_l = 0;
     uint64_t work = 0;
-    if (FILL && lane < nq) cur_l = offsets[qid_l];
+    if (PASS == PASS_FILL && lane < nq) cur_l = a.offsets[qid_l];
+    uint32_t scur = 0, scap = 0;
+    uint64_t sbase = 0;
+    if (PASS == PASS_SLOT) {
+        scap = uniform(a.slotCap[tile]);
+        sbase = uniform64(a.slotOff[tile]);
+    }
     bool overflow = false;
 
-    for (uint32_t cw = 0; cw < ncombo; cw += GDSJ_WAVE) {
-        const uint32_t nrw = uniform(ncombo - cw < (uint32_t)GDSJ_WAVE ? ncombo - cw : (uint32_t)GDSJ_WAVE);
-        uint2 run = lane < nrw ? combo_run(cw + lane, cc, glo, ghi, ga, cellStart, hkeys, hvals)
-                               : make_uint2(0u, 0u);
+    for (uint32_t cw = 0; cw < nwin; cw += GDSJ_WAVE) {
+        const uint32_t nrw = uniform(nwin - cw < (uint32_t)GDSJ_WAVE ? nwin - cw : (uint32_t)GDSJ_WAVE);
+        uint2 run = make_uint2(0u, 0u);
+        if (lane < nrw)
+            run = pre ? a.runs[(uint64_t)cell * a.runStride + cw + lane]
+                      : combo_run(cw + lane, cc, glo, ghi, ga, a.cellStart, a.hkeys, a.hvals);
         uint32_t len = run.y - run.x;
         uint32_t incl = len;
 #pragma unroll
@@ -296,7 +218,7 @@ __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const ui
 #pragma unroll
                     for (int l = 0; l < (D > 0 ? D : 1); l++) c[ch][l] = xs[(uint64_t)l * ld + sp];
                 }
-                if (FILL) cid[ch] = (ch < (int)nch) ? ids[sp] : 0u;
+                if (NEED_IDS) cid[ch] = (ch < (int)nch) ? ids[sp] : 0u;
             }
             for (uint32_t qi = 0; qi < nq; qi++) {
                 const uint32_t qpos = q0 + qi;
@@ -307,7 +229,7 @@ __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const ui
                 }
                 uint32_t tot = 0;
                 uint64_t cur = 0;
-                if (FILL) {
+                if (PASS == PASS_FILL) {
                     uint32_t lo = __builtin_amdgcn_readlane((uint32_t)cur_l, qi);
                     uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(cur_l >> 32), qi);
                     cur = ((uint64_t)hi << 32) | lo;
@@ -319,25 +241,36 @@ __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const ui
                         if (D > 0) {
                             bool alive;
                             T acc = dist_reg<T, (D > 0 ? D : 1), MODE, (ILPV > 0 ? ILPV : 1)>(
-                                q, c[ch], thr, valid[ch], sc != 0, alive);
+                                q, c[ch], thr, valid[ch], a.sc != 0, alive);
                             hit = alive && valid[ch] && acc <= thr;
                         } else {
-                            T acc = dist_mem<T>(xs, ld, qpos, cpos[ch], dd, MODE, ilp_rt);
+                            T acc = dist_mem<T>(xs, ld, qpos, cpos[ch], a.d, MODE, a.ilp);
                             hit = valid[ch] && acc <= thr;
                         }
                         const uint64_t m = ballot(hit);
-                        if (FILL) {
+                        const uint32_t pm = popc64(m);
+                        if (PASS == PASS_FILL) {
                             if (hit) {
                                 const uint64_t pos = cur + mbcnt(m);
-                                if (pos < cap) out[pos] = cid[ch]; else overflow = true;
+                                if (pos < a.cap) a.out[pos] = cid[ch]; else overflow = true;
                             }
-                            cur += popc64(m);
+                            cur += pm;
                         } else {
-                            tot += popc64(m);
+                            if (PASS == PASS_SLOT && pm) {
+                                if (hit) {
+                                    const uint32_t p = scur + mbcnt(m);
+                                    if (p < scap) {
+                                        a.slotIds[sbase + p] = cid[ch];
+                                        a.slotQ[sbase + p] = (uint8_t)qi;
+                                    }
+                                }
+                                scur += pm;
+                            }
+                            tot += pm;
                         }
                     }
                 }
-                if (FILL) {
+                if (PASS == PASS_FILL) {
                     if (lane == qi) cur_l = cur;
                 } else {
                     if (lane == qi) cnt_l += tot;
@@ -345,9 +278,13 @@ __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const ui
             }
         }
     }
-    if (!FILL && lane < nq) counts[qid_l] = cnt_l;
-    if (!FILL && tileWork && lane == 0) tileWork[tile] = work * nq;
-    if (FILL && overflow) atomicOr(status, 1u);
+    if (PASS != PASS_FILL && lane < nq) a.counts[qid_l] = cnt_l;
+    if (PASS != PASS_FILL && a.tileWork && lane == 0) a.tileWork[tile] = work * nq;
+    if (PASS == PASS_SLOT && lane == 0) {
+        a.slotUsed[tile] = scur;
+        if (scur > scap) a.ovfList[atomicAdd(a.ovfCount, 1u)] = tile;
+    }
+    if (PASS == PASS_FILL && overflow) atomicOr(a.status, 1u);
 }
 
 }  // namespace gdsj

@@ -1,0 +1,106 @@
+// neighbors.h -- adjacent-cell enumeration shared by the run-list kernel
+// and the join kernels: the reference's candidate cells (kernel.cu:180-300
+// with kernelloops.h) found through the cell lookup table, plus the
+// query-box pruning that skips cells no query of a tile can reach.
+#pragma once
+#include "common.h"
+#include "grid.h"
+
+namespace gdsj {
+
+__device__ __forceinline__ uint32_t shfl_u32(uint32_t v, uint32_t src) { return __shfl(v, (int)src, GDSJ_WAVE); }
+
+// lane -> candidate sorted position for chunk-relative index g within a
+// window of up to 64 runs (run i held by lane i: start, exclusive prefix)
+__device__ __forceinline__ uint32_t cand_pos(uint32_t g, uint32_t nrw, uint32_t rstart, uint32_t rexcl) {
+    uint32_t idx = 0;
+#pragma unroll
+    for (uint32_t step = 32; step > 0; step >>= 1) {
+        uint32_t cnd = idx + step;
+        uint32_t e = shfl_u32(rexcl, cnd < 64 ? cnd : 63);
+        if (cnd < nrw && e <= g) idx = cnd;
+    }
+    return shfl_u32(rstart, idx) + (g - shfl_u32(rexcl, idx));
+}
+
+// Neighbor run of combination t (digits of t in base 3 = offsets -1/0/+1 of
+// dims 1..k-1, dim k-1 most significant, so runs come in ascending linear
+// id): the up to three cells (c0-1, c0, c0+1) of that row, which are
+// consecutive in the sorted cell order.  Coordinates outside
+// [0, nCells-1] are skipped, as the reference's clamps do (kernel.cu:183-187).
+// Cells whose squared gap to the tile's query box exceeds prune_lim are
+// skipped too: no pair with them can pass the predicate.
+__device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK], const double *glo,
+                                           const double *ghi, const GridArgs &ga,
+                                           const uint32_t *__restrict__ cellStart,
+                                           const uint64_t *__restrict__ hkeys,
+                                           const uint32_t *__restrict__ hvals) {
+    uint64_t base = 0;
+    bool ok = true;
+    double gap = 0.0;
+    uint32_t tt = t;
+    for (uint32_t j = 1; j < ga.k; j++) {
+        const uint32_t dlt = tt % 3u;
+        tt /= 3u;
+        const int64_t cj = (int64_t)cc[j] + (int64_t)dlt - 1;
+        ok &= (cj >= 0) && (cj < (int64_t)ga.nCells[j]);
+        gap += dlt == 0 ? glo[j] : (dlt == 2 ? ghi[j] : 0.0);
+        base += (uint64_t)cj * ga.mult[j];
+    }
+    uint2 run = make_uint2(0u, 0u);
+    if (!ok || gap > ga.prune_lim) return run;
+    uint32_t x0 = cc[0] == 0 ? 0u : cc[0] - 1u;
+    uint32_t x1 = cc[0] + 1u < ga.nCells[0] ? cc[0] + 1u : ga.nCells[0] - 1u;
+    if (x0 < cc[0] && gap + glo[0] > ga.prune_lim) x0 = cc[0];
+    if (x1 > cc[0] && gap + ghi[0] > ga.prune_lim) x1 = cc[0];
+    int32_t first = -1, last = -1;
+    for (uint32_t x = x0; x <= x1; x++) {
+        const int32_t ci = cell_find(hkeys, hvals, ga.hbits, base + x);
+        if (ci >= 0) {
+            if (first < 0) first = ci;
+            last = ci;
+        }
+    }
+    if (first >= 0) run = make_uint2(cellStart[first], cellStart[last + 1]);
+    return run;
+}
+
+// Squared gaps from the bounding box of the n query points at sorted
+// positions [q0, q0+n) to the faces of their cell in each indexed dim:
+// glo[j] towards coordinate c_j - 1, ghi[j] towards c_j + 1.  Face
+// positions are widened by the classification rounding of (x - min) / eps
+// (a few units of DTYPE roundoff), so the gaps never overstate the distance
+// to any point of the neighbor cell.  Written by lane 0 (wave-uniform).
+template <typename T>
+__device__ __forceinline__ void tile_gaps(const T *__restrict__ xs, uint64_t ld, uint32_t q0, uint32_t n,
+                                          const uint32_t (&cc)[MAXK], const GridArgs &ga, double *glo,
+                                          double *ghi) {
+    const uint32_t lane = lane_id();
+    for (uint32_t j = 0; j < ga.k; j++) {
+        double mn = 1e300, mx = -1e300;
+        for (uint32_t b = 0; b < n; b += GDSJ_WAVE) {
+            if (b + lane < n) {
+                const double v = (double)xs[(uint64_t)j * ld + q0 + b + lane];
+                mn = fmin(mn, v);
+                mx = fmax(mx, v);
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = fmin(mn, __shfl_xor(mn, off, GDSJ_WAVE));
+            mx = fmax(mx, __shfl_xor(mx, off, GDSJ_WAVE));
+        }
+        const double lo_face = ga.minArr[j] + (double)cc[j] * ga.eps;
+        const double hi_face = lo_face + ga.eps;
+        const double tol = 8.0 * ga.ulp * (fabs(ga.minArr[j]) + fabs(hi_face) + 2.0 * ga.eps);
+        const double gl = fmax(0.0, mn - lo_face - tol);
+        const double gh = fmax(0.0, hi_face - tol - mx);
+        if (lane == 0) {
+            glo[j] = gl * gl;
+            ghi[j] = gh * gh;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+}  // namespace gdsj

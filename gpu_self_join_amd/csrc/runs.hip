@@ -1,0 +1,142 @@
+// runs.hip -- per-cell neighbor-run lists and slot compaction.
+//
+// build_runs: one wave per non-empty cell enumerates the 3^(k-1) rows of
+// adjacent cells once (lookup table, clamps of kernel.cu:183-187, pruning
+// against the cell's point box) and stores the non-empty runs compacted at
+// a fixed stride, so the join passes read them instead of re-enumerating.
+//
+// compact_slots: one wave per tile moves the tile's slot entries (neighbor
+// id + query index, written in candidate order) into the CSR rows with a
+// stable partition by query index.
+#include "join.h"
+#include "neighbors.h"
+
+namespace gdsj {
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uint64_t ld, GridArgs ga,
+                                                    const uint32_t *__restrict__ cellStart,
+                                                    const uint32_t *__restrict__ cellCoord,
+                                                    const uint64_t *__restrict__ hkeys,
+                                                    const uint32_t *__restrict__ hvals, uint32_t ncombo,
+                                                    uint64_t stride, int prune, uint2 *__restrict__ runs,
+                                                    uint32_t *__restrict__ nruns, uint32_t *__restrict__ cellCand) {
+    const uint32_t lane = lane_id();
+    const uint32_t cell = uniform((uint32_t)(blockIdx.x * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
+    if (cell >= ga.ncell) return;
+    __shared__ double s_gap[256 / GDSJ_WAVE][2][MAXK];
+    double *glo = s_gap[threadIdx.x / GDSJ_WAVE][0];
+    double *ghi = s_gap[threadIdx.x / GDSJ_WAVE][1];
+    uint32_t cc[MAXK];
+    for (uint32_t j = 0; j < ga.k; j++) cc[j] = uniform(cellCoord[(uint64_t)j * ga.ncell + cell]);
+    const uint32_t s0 = uniform(cellStart[cell]), s1 = uniform(cellStart[cell + 1]);
+    if (prune) {
+        tile_gaps<T>(xs, ld, s0, s1 - s0, cc, ga, glo, ghi);
+    } else {
+        if (lane < MAXK) {
+            glo[lane] = 0.0;
+            ghi[lane] = 0.0;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    uint32_t cursor = 0;
+    uint32_t cand = 0;
+    uint2 *mine = runs + (uint64_t)cell * stride;
+    for (uint32_t cw = 0; cw < ncombo; cw += GDSJ_WAVE) {
+        uint2 run = make_uint2(0u, 0u);
+        if (cw + lane < ncombo) run = combo_run(cw + lane, cc, glo, ghi, ga, cellStart, hkeys, hvals);
+        const bool ne = run.y > run.x;
+        const uint64_t m = ballot(ne);
+        if (ne) mine[cursor + mbcnt(m)] = run;
+        cursor += popc64(m);
+        uint32_t len = run.y - run.x;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) len += __shfl_xor(len, off, GDSJ_WAVE);
+        cand += uniform(len);
+    }
+    if (lane == 0) {
+        nruns[cell] = cursor;
+        cellCand[cell] = cand;
+    }
+}
+
+int build_runs(Grid &G, hipStream_t s, bool prune, bool *built) {
+    *built = false;
+    uint64_t ncombo = 1;
+    for (uint32_t j = 1; j < G.k; j++) ncombo *= 3;
+    const uint64_t entries = (uint64_t)G.ncell * ncombo;
+    if (G.ncell == 0 || entries * 8 > (6ull << 30)) return GDSJ_OK;
+    G.runStride = ncombo;
+    GDSJ_TRY(G.runs.ensure(entries * 8 + 64));
+    GDSJ_TRY(G.nruns.ensure((size_t)G.ncell * 4 + 64));
+    GDSJ_TRY(G.cellCand.ensure((size_t)G.ncell * 4 + 64));
+    const uint32_t blocks = (G.ncell + 3) / 4;
+    if (G.dtype == GDSJ_DTYPE_F32)
+        k_cell_runs<float><<<blocks, 256, 0, s>>>(G.xs.as<float>(), G.ld, G.ga, G.cellStart.as<uint32_t>(),
+                                                  G.cellCoord.as<uint32_t>(), G.hkeys.as<uint64_t>(),
+                                                  G.hvals.as<uint32_t>(), (uint32_t)ncombo, ncombo, prune ? 1 : 0,
+                                                  G.runs.as<uint2>(), G.nruns.as<uint32_t>(),
+                                                  G.cellCand.as<uint32_t>());
+    else
+        k_cell_runs<double><<<blocks, 256, 0, s>>>(G.xs.as<double>(), G.ld, G.ga, G.cellStart.as<uint32_t>(),
+                                                   G.cellCoord.as<uint32_t>(), G.hkeys.as<uint64_t>(),
+                                                   G.hvals.as<uint32_t>(), (uint32_t)ncombo, ncombo,
+                                                   prune ? 1 : 0, G.runs.as<uint2>(), G.nruns.as<uint32_t>(),
+                                                   G.cellCand.as<uint32_t>());
+    GDSJ_HIP(hipGetLastError());
+    *built = true;
+    return GDSJ_OK;
+}
+
+__global__ __launch_bounds__(256) void k_compact(const JoinArgs a) {
+    __shared__ uint32_t scnt[256 / GDSJ_WAVE][GDSJ_WAVE];
+    const uint32_t lane = lane_id();
+    const uint32_t w = threadIdx.x / GDSJ_WAVE;
+    const uint32_t tile = uniform((uint32_t)(blockIdx.x * (blockDim.x / GDSJ_WAVE) + w));
+    if (tile >= a.ntiles) return;
+    const uint32_t used = uniform(a.slotUsed[tile]);
+    if (used > uniform(a.slotCap[tile])) return;  // overflowed: completed by the fill pass
+    const uint4 tl = a.tiles[tile];
+    const uint32_t q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
+    const uint64_t sbase = uniform64(a.slotOff[tile]);
+    uint64_t cur_l = lane < nq ? a.offsets[a.ids[q0 + lane]] : 0ull;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    scnt[w][lane] = 0;
+    __builtin_amdgcn_wave_barrier();
+    bool overflow = false;
+    for (uint32_t e = 0; e < used; e += GDSJ_WAVE) {
+        const bool valid = e + lane < used;
+        const uint32_t qi = valid ? (uint32_t)a.slotQ[sbase + e + lane] : 0u;
+        const uint32_t id = valid ? a.slotIds[sbase + e + lane] : 0u;
+        uint64_t peers = ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 6; b++) {
+            const bool bit = (qi >> b) & 1u;
+            const uint64_t mb = ballot(bit);
+            peers &= bit ? mb : ~mb;
+        }
+        const uint64_t below = peers & lt;
+        const uint32_t lo = __shfl((uint32_t)cur_l, (int)qi, GDSJ_WAVE);
+        const uint32_t hi = __shfl((uint32_t)(cur_l >> 32), (int)qi, GDSJ_WAVE);
+        const uint64_t pos = (((uint64_t)hi << 32) | lo) + popc64(below);
+        if (valid) {
+            if (pos < a.cap) a.out[pos] = id; else overflow = true;
+            if (below == 0) scnt[w][qi] = popc64(peers);
+        }
+        __builtin_amdgcn_wave_barrier();
+        cur_l += scnt[w][lane];
+        scnt[w][lane] = 0;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (overflow) atomicOr(a.status, 1u);
+}
+
+int compact_slots(const JoinArgs &a, hipStream_t s) {
+    const uint32_t blocks = (a.ntiles + 3) / 4;
+    if (blocks == 0) return GDSJ_OK;
+    k_compact<<<blocks, 256, 0, s>>>(a);
+    GDSJ_HIP(hipGetLastError());
+    return GDSJ_OK;
+}
+
+}  // namespace gdsj
