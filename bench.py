@@ -48,6 +48,43 @@ def cpu_baseline(x, eps, params, sample_frac, seed=123):
                       "extrapolated linearly in queries" % (nq, n, 100.0 * nq / n, pairs_sample, wall)}
 
 
+EST = {"exact": 0, "learned": 1, "sample": 2}
+
+
+def run_variant(sj, pts, eps, kw, steps, warmup, total=None):
+    """Times `steps` joins (after `warmup`) with output into preallocated buffers."""
+    import torch
+    n = pts.shape[0]
+    dev = pts.device
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    if total is None:
+        total = sj.join(pts, eps, counts=counts, offsets=offsets, stream=stream.cuda_stream, **kw).total
+    ids = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+
+    def step():
+        res = sj.join(pts, eps, out_ids=ids, counts=counts, offsets=offsets, stream=stream.cuda_stream, **kw)
+        assert res.total == total
+        return res.stats
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    per = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        per.append(step())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    del ids
+    return total, 1e3 * wall / steps, per
+
+
+PHASES = ("ms_grid", "ms_estimate", "ms_count", "ms_scan", "ms_compact", "ms_fill", "ms_sort", "ms_train",
+          "ms_total")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +93,8 @@ def main():
     ap.add_argument("--workload", default="gmm6d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=float, default=0.01)
+    ap.add_argument("--variants", default="exact,learned",
+                    help="estimators to time (exact = counting pass + fill; learned = MLP-sized single pass)")
     ap.add_argument("--param", action="append", default=[],
                     help="override a gdsj_params field that does not change the result, e.g. queryreorder=0")
     args = ap.parse_args()
@@ -76,78 +115,75 @@ def main():
     n, d = x.shape
     dev = torch.device("cuda:0")
     pts = torch.from_numpy(x).to(dev)
-    sj = SelfJoin(0)
-    kw = dict(k=params["k"], ilp=params["ilp"], shortcircuit=params["shortcircuit"], reorder=params["reorder"],
-              stamp=params["stamp"], queryreorder=params["queryreorder"])
+    base = dict(k=params["k"], ilp=params["ilp"], shortcircuit=params["shortcircuit"], reorder=params["reorder"],
+                stamp=params["stamp"], queryreorder=params["queryreorder"])
     for item in args.param:
         key, val = item.split("=")
-        if key in ("k", "ilp", "reorder"):
+        if key in ("k", "ilp", "reorder", "shortcircuit"):
             raise SystemExit("--param %s would change the workload's result contract" % key)
-        kw[key] = int(val)
-    counts = torch.empty(n, dtype=torch.int32, device=dev)
-    offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
-    r = sj.join(pts, eps, counts=counts, offsets=offsets, **kw)
-    total = r.total
-    ids = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream(dev)
-
-    def step():
-        res = sj.join(pts, eps, out_ids=ids, counts=counts, offsets=offsets, stream=stream.cuda_stream, **kw)
-        assert res.total == total
-        return res.stats
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    per = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        per.append(step())
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = 1e3 * wall / args.steps
-
-    def avg(key):
-        return float(np.mean([s[key] for s in per]))
-
-    phase = {k: avg(k) for k in ("ms_grid", "ms_count", "ms_scan", "ms_fill", "ms_sort", "ms_total")}
-    s = per[-1]
+        base[key] = int(val)
     esz = np.dtype(w["dtype"]).itemsize
-    # algorithmic bytes per launch: points read once (N*D*s); the count kernel
-    # writes N uint32 counts, the fill kernel writes P uint32 neighbor ids
+    results = {}
+    total = None
+    for vname in args.variants.split(","):
+        kw = dict(base, estimator=EST[vname])
+        sj = SelfJoin(0)
+        if vname == "learned":
+            kw["train"] = 0  # trains on the first join's counts, then sizes slots
+        total, ms, per = run_variant(sj, pts, eps, kw, args.steps, args.warmup, total)
+        sj.close()
+        phase = {k: float(np.mean([s[k] for s in per])) for k in PHASES}
+        last = per[-1]
+        results[vname] = {"ms_per_step": ms, "pairs_per_s": total / (ms * 1e-3), "phase_ms": phase, "stats": last}
+    best = min(results, key=lambda v: results[v]["ms_per_step"])
+    r = results[best]
+    phase = r["phase_ms"]
+    s = r["stats"]
+    # dominant kernel and its algorithmic bytes per launch: every pass reads
+    # the points once (N*D*s); the counting pass writes N counts, the slotted
+    # pass N counts + 5 bytes per pair of slot, the fill pass 4 bytes per pair
     kb = {"count": n * d * esz + 4 * n, "fill": n * d * esz + 4 * total}
     kms = {"count": phase["ms_count"], "fill": phase["ms_fill"]}
+    kname = {"count": "k_join<count pass>", "fill": "k_join<fill pass>"}
+    if best != "exact":
+        kb["count"] = n * d * esz + 4 * n + 5 * total
+        kname["count"] = "k_join<slotted pass>"
+        kb["compact"] = 9 * total
+        kms["compact"] = phase["ms_compact"]
+        kname["compact"] = "k_compact"
     dom = max(kms, key=kms.get)
     achieved = kb[dom] / (kms[dom] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": "k_join<%s pass>" % dom, "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+    roof = {"bound": "hbm", "kernel": kname[dom], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "alg_bytes_per_launch": kb[dom], "avg_launch_ms": round(kms[dom], 4)}
-    value = total / (ms_step * 1e-3)
+    variants = {}
+    for vname, vr in results.items():
+        st = vr["stats"]
+        variants[vname] = {"ms_per_step": round(vr["ms_per_step"], 4), "pairs_per_s": vr["pairs_per_s"],
+                           "phase_ms": {k: round(v, 4) for k, v in vr["phase_ms"].items()}}
+        if vname != "exact":
+            variants[vname]["estimator"] = {k: st[k] for k in ("est_predicted", "est_capacity", "est_overflow_tiles",
+                                                                "est_tiles", "est_mean_rel_err", "est_max_rel_err")}
     out = {
-        "metric": "exact result pairs/s (plain grid join)",
-        "value": value,
+        "metric": "exact result pairs/s (grid self-join, %s)" % best,
+        "value": r["pairs_per_s"],
         "unit": "pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(r["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if esz == 4 else "f64",
         "data": "synthetic (seeded %s generator)" % name,
         "config": {"workload": name, "n": n, "dims": d, "eps": eps, "total_pairs": total,
-                   "pairs_per_point": total / n, **params, **{k: v for k, v in kw.items() if k not in params}},
-        "phase_ms": {k: round(v, 4) for k, v in phase.items()},
-        "grid": {"nonempty_cells": s["n_nonempty_cells"], "tiles": s["n_tiles"], "runs": s["n_runs"],
-                 "candidates": s["candidates"]},
+                   "pairs_per_point": total / n, **base, "estimator": best},
+        "variants": variants,
+        "grid": {"nonempty_cells": s["n_nonempty_cells"], "tiles": s["n_tiles"], "candidates": s["candidates"]},
         "roofline": roof,
     }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(x, eps, params, args.cpu_sample)
-    else:
-        out["cpu_baseline"] = None
-    sj.close()
+    out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(x, eps, params, args.cpu_sample)
     print(json.dumps(out))
 
 

@@ -30,6 +30,8 @@ EXPORTS = [
     "gdsj_join_host", "gdsj_result_copy", "gdsj_get_stats", "gdsj_eps_threshold_f32",
     "gdsj_eps_threshold_f64", "gdsj_last_error", "GDSJoinPy", "GDSJoinPyDouble",
     "copyResultIntoPythonArray", "gdsj_set_legacy_params", "gdsj_legacy_status",
+    "gdsj_mlp_param_count", "gdsj_mlp_feature_count", "gdsj_mlp_forward", "gdsj_mlp_grad",
+    "gdsj_model_get", "gdsj_model_set",
 ]
 
 
@@ -38,16 +40,19 @@ class GdsjParams(C.Structure):
                 ("reorder", C.c_uint32), ("stamp", C.c_uint32), ("queryreorder", C.c_uint32),
                 ("filter", C.c_uint32), ("estimator", C.c_uint32), ("sort_neighbors", C.c_uint32),
                 ("verbose", C.c_uint32), ("tile_queries", C.c_uint32), ("schedule", C.c_uint32),
-                ("prune", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("prune", C.c_uint32), ("train", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
 class GdsjStats(C.Structure):
     _fields_ = [("total_pairs", C.c_uint64), ("candidates", C.c_uint64), ("n_nonempty_cells", C.c_uint64),
                 ("total_cells", C.c_uint64), ("n_tiles", C.c_uint64), ("n_runs", C.c_uint64),
-                ("filter_rechecks", C.c_uint64), ("filter_band", C.c_uint64), ("est_predicted", C.c_uint64),
-                ("est_overflow_tiles", C.c_uint64), ("ms_grid", C.c_double), ("ms_count", C.c_double),
-                ("ms_scan", C.c_double), ("ms_fill", C.c_double), ("ms_sort", C.c_double),
-                ("ms_total", C.c_double), ("perm", C.c_uint32 * 64)]
+                ("filter_rechecks", C.c_uint64), ("filter_hits", C.c_uint64), ("est_predicted", C.c_uint64),
+                ("est_capacity", C.c_uint64), ("est_overflow_tiles", C.c_uint64), ("est_tiles", C.c_uint64),
+                ("est_mean_rel_err", C.c_double), ("est_max_rel_err", C.c_double), ("filter_max_err", C.c_double),
+                ("ms_grid", C.c_double), ("ms_estimate", C.c_double), ("ms_count", C.c_double),
+                ("ms_scan", C.c_double), ("ms_compact", C.c_double), ("ms_fill", C.c_double),
+                ("ms_sort", C.c_double), ("ms_train", C.c_double), ("ms_total", C.c_double),
+                ("perm", C.c_uint32 * 64)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "perm"}
@@ -101,6 +106,12 @@ def load():
     L.copyResultIntoPythonArray.restype = None
     L.gdsj_set_legacy_params.argtypes = [C.POINTER(GdsjParams)]
     L.gdsj_legacy_status.restype = C.c_int
+    L.gdsj_mlp_param_count.restype = u32
+    L.gdsj_mlp_feature_count.restype = u32
+    L.gdsj_mlp_forward.argtypes = [P, P, u32, P]
+    L.gdsj_mlp_grad.argtypes = [P, P, P, u32, P, P]
+    L.gdsj_model_get.argtypes = [P, P, u32, C.POINTER(C.c_double)]
+    L.gdsj_model_set.argtypes = [P, P, u32, C.c_double]
     _lib = L
     return L
 

@@ -17,6 +17,7 @@
 #include <mutex>
 #include <vector>
 
+#include "estimator.h"
 #include "grid.h"
 #include "join.h"
 #include "scan.h"
@@ -43,6 +44,11 @@ struct gdsj_handle {
     hipStream_t stream = nullptr;
     Grid G;
     DevBuf counts, offsets, ids, data, scanTmp, status, work;
+    // slotted single pass (estimator-sized per-tile result slots)
+    DevBuf slotCap, slotOff, slotUsed, slotOvf, slotIds, slotQ, scalars, estPred, estScratch;
+    Estimator est;
+    bool slotted = false;
+    uint32_t novf = 0;
     gdsj_params p{};
     int mode = 0;
     double thr = 0;
@@ -77,6 +83,7 @@ int gdsj_params_default(gdsj_params *p) {
     p->tile_queries = 0;
     p->schedule = 0;
     p->prune = 0;
+    p->train = 0;
     return GDSJ_OK;
 }
 
@@ -134,7 +141,10 @@ int gdsj_destroy(gdsj_handle *h) {
     (void)hipSetDevice(h->device);
     if (h->own) (void)hipStreamSynchronize(h->own);
     h->G.release();
-    DevBuf *b[] = {&h->counts, &h->offsets, &h->ids, &h->data, &h->scanTmp, &h->status, &h->work};
+    DevBuf *b[] = {&h->counts,  &h->offsets,  &h->ids,     &h->data,    &h->scanTmp, &h->status,
+                   &h->work,    &h->slotCap,  &h->slotOff, &h->slotUsed, &h->slotOvf, &h->slotIds,
+                   &h->slotQ,   &h->scalars,  &h->estPred, &h->estScratch};
+    h->est.release();
     for (DevBuf *x : b) x->release();
     for (int i = 0; i < 8; i++)
         if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
@@ -194,6 +204,19 @@ static JoinArgs make_args(gdsj_handle *h) {
     return a;
 }
 
+// Writes the table of the last join into d_ids: the exact fill pass, or,
+// after a slotted pass, compaction of the slots plus the fill pass for the
+// tiles whose slot overflowed.
+static void set_slot_args(gdsj_handle *h, JoinArgs &a) {
+    a.slotOff = h->slotOff.as<uint64_t>();
+    a.slotCap = h->slotCap.as<uint32_t>();
+    a.slotUsed = h->slotUsed.as<uint32_t>();
+    a.slotIds = h->slotIds.as<uint32_t>();
+    a.slotQ = h->slotQ.as<uint8_t>();
+    a.ovfList = h->slotOvf.as<uint32_t>();
+    a.ovfCount = h->scalars.as<uint32_t>();
+}
+
 static int do_fill(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
     hipStream_t s = h->stream;
     JoinArgs a = make_args(h);
@@ -202,7 +225,19 @@ static int do_fill(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
     a.cap = cap;
     GDSJ_HIP(hipMemsetAsync(h->status.p, 0, 4, s));
     GDSJ_HIP(hipEventRecord(h->ev[4], s));
-    GDSJ_TRY(join_pass(h->G, a, h->mode, PASS_FILL, s));
+    if (h->slotted) {
+        set_slot_args(h, a);
+        GDSJ_TRY(compact_slots(a, s));
+        GDSJ_HIP(hipEventRecord(h->ev[7], s));
+        if (h->novf) {
+            a.tileList = h->slotOvf.as<uint32_t>();
+            a.ntiles = h->novf;
+            GDSJ_TRY(join_pass(h->G, a, h->mode, PASS_FILL, s));
+        }
+    } else {
+        GDSJ_TRY(join_pass(h->G, a, h->mode, PASS_FILL, s));
+        GDSJ_HIP(hipEventRecord(h->ev[7], s));
+    }
     GDSJ_HIP(hipEventRecord(h->ev[5], s));
     if (h->p.sort_neighbors)
         GDSJ_TRY(sort_rows(h->cur_offsets, h->G.n, d_ids, s));
@@ -215,8 +250,57 @@ static int do_fill(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
         return GDSJ_EDEVICE;
     }
     h->have_table = true;
-    h->st.ms_fill = ms_between(h->ev[4], h->ev[5]);
+    if (h->slotted) {
+        h->st.ms_compact = ms_between(h->ev[4], h->ev[7]);
+        h->st.ms_fill = ms_between(h->ev[7], h->ev[5]);
+    } else {
+        h->st.ms_fill = ms_between(h->ev[4], h->ev[5]);
+    }
     h->st.ms_sort = ms_between(h->ev[5], h->ev[6]);
+    return GDSJ_OK;
+}
+
+// Single pass with per-tile result slots sized by an estimator.  Returns
+// GDSJ_OK with *used = true when the slotted pass ran (counts written).
+static int slotted_pass(gdsj_handle *h, JoinArgs &a, bool *used) {
+    *used = false;
+    hipStream_t s = h->stream;
+    Grid &G = h->G;
+    const uint32_t nt = G.ntiles;
+    if (!G.haveRuns || nt == 0) return GDSJ_OK;
+    const bool learned = h->p.estimator == GDSJ_EST_LEARNED;
+    if (learned && !h->est.trained) return GDSJ_OK;  // untrained: the exact passes (and training) run
+    GDSJ_TRY(h->slotCap.ensure((size_t)nt * 4 + 64));
+    GDSJ_TRY(h->slotOff.ensure(((size_t)nt + 1) * 8 + 64));
+    GDSJ_TRY(h->slotUsed.ensure((size_t)nt * 4 + 64));
+    GDSJ_TRY(h->slotOvf.ensure((size_t)nt * 4 + 64));
+    GDSJ_TRY(h->estPred.ensure((size_t)nt * 4 + 64));
+    uint64_t predicted = 0;
+    GDSJ_HIP(hipEventRecord(h->ev[1], s));
+    if (learned) {
+        GDSJ_TRY(estimator_capacity_learned(h->est, G, h->slotCap.as<uint32_t>(), &predicted, s));
+        GDSJ_HIP(hipMemcpyAsync(h->estPred.p, h->est.pred.p, (size_t)nt * 4, hipMemcpyDeviceToDevice, s));
+    } else {
+        GDSJ_TRY(estimator_capacity_sampling(G, &a, h->mode, a.counts, h->slotCap.as<uint32_t>(), &predicted,
+                                             h->estPred.as<float>(), s));
+    }
+    GDSJ_TRY(h->scanTmp.ensure(scan_tmp_bytes((uint64_t)nt + 1) + scan_tmp_bytes((uint64_t)G.n + 1) + 256));
+    GDSJ_TRY(exclusive_scan_u32_u64(h->slotCap.as<uint32_t>(), h->slotOff.as<uint64_t>(), nt, true,
+                                    h->scanTmp.p, s));
+    uint64_t slots = 0;
+    GDSJ_HIP(hipMemcpyAsync(&slots, h->slotOff.as<uint64_t>() + nt, 8, hipMemcpyDeviceToHost, s));
+    GDSJ_HIP(hipStreamSynchronize(s));
+    h->st.est_predicted = predicted;
+    h->st.est_capacity = slots;
+    h->st.est_tiles = nt;
+    GDSJ_TRY(h->slotIds.ensure((size_t)slots * 4 + 256));
+    GDSJ_TRY(h->slotQ.ensure((size_t)slots + 256));
+    GDSJ_TRY(h->scalars.ensure(64));
+    GDSJ_HIP(hipMemsetAsync(h->scalars.p, 0, 4, s));
+    GDSJ_HIP(hipEventRecord(h->ev[2], s));
+    set_slot_args(h, a);
+    GDSJ_TRY(join_pass(G, a, h->mode, PASS_SLOT, s));
+    *used = true;
     return GDSJ_OK;
 }
 
@@ -230,6 +314,8 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     }
     GDSJ_HIP(hipSetDevice(h->device));
     h->have_counts = h->have_table = false;
+    h->slotted = false;
+    h->novf = 0;
     GDSJ_TRY(resolve_params(pin, dims, &h->p));
     h->stream = stream ? (hipStream_t)stream : h->own;
     hipStream_t s = h->stream;
@@ -256,7 +342,6 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
         GDSJ_TRY(build_runs(h->G, s, prune, &built));
         h->G.haveRuns = built;
     }
-    // ---- counting pass + CSR offsets
     uint64_t *offs = d_offsets;
     if (!offs) {
         GDSJ_TRY(h->offsets.ensure(((size_t)n + 1) * 8));
@@ -269,18 +354,28 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     GDSJ_TRY(h->scanTmp.ensure(scan_tmp_bytes((uint64_t)n + 1) + 256));
     GDSJ_TRY(h->work.ensure((size_t)h->G.ntiles * 8 + 256));
     a.tileWork = h->work.as<uint64_t>();
-    GDSJ_HIP(hipEventRecord(h->ev[1], s));
-    GDSJ_TRY(join_pass(h->G, a, h->mode, PASS_COUNT, s));
-    GDSJ_HIP(hipEventRecord(h->ev[2], s));
-    GDSJ_TRY(exclusive_scan_u32_u64(d_counts, offs, n, true, h->scanTmp.p, s));
+    // ---- counting: slotted single pass when an estimator sizes the slots,
+    //      else the exact counting pass
+    bool slotted = false;
+    if (h->p.estimator != GDSJ_EST_EXACT) GDSJ_TRY(slotted_pass(h, a, &slotted));
+    if (!slotted) {
+        GDSJ_HIP(hipEventRecord(h->ev[1], s));
+        GDSJ_HIP(hipEventRecord(h->ev[2], s));
+        GDSJ_TRY(join_pass(h->G, a, h->mode, PASS_COUNT, s));
+    }
+    h->slotted = slotted;
     GDSJ_HIP(hipEventRecord(h->ev[3], s));
+    GDSJ_TRY(exclusive_scan_u32_u64(d_counts, offs, n, true, h->scanTmp.p, s));
+    GDSJ_HIP(hipEventRecord(h->ev[4], s));
     uint64_t *candSum = h->status.as<uint64_t>() + 4;
     GDSJ_TRY(sum_u64(a.tileWork, h->G.ntiles, candSum, s));
-    uint64_t hres[2] = {0, 0};
+    uint64_t hres[3] = {0, 0, 0};
     GDSJ_HIP(hipMemcpyAsync(&hres[0], offs + n, 8, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipMemcpyAsync(&hres[1], candSum, 8, hipMemcpyDeviceToHost, s));
+    if (slotted) GDSJ_HIP(hipMemcpyAsync(&hres[2], h->scalars.p, 4, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipStreamSynchronize(s));
     uint64_t tot = hres[0];
+    h->novf = (uint32_t)hres[2];
     h->total = tot;
     *total = tot;
     h->have_counts = true;
@@ -289,24 +384,37 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     h->st.n_nonempty_cells = h->G.ncell;
     h->st.total_cells = h->G.totalCells;
     h->st.n_tiles = h->G.ntiles;
-    h->st.n_runs = (uint64_t)h->G.ntiles * a.ncombo;
+    h->st.n_runs = h->G.haveRuns ? 0 : (uint64_t)h->G.ntiles * a.ncombo;
+    h->st.est_overflow_tiles = h->novf;
     for (uint32_t i = 0; i < dims && i < 64; i++) h->st.perm[i] = h->G.perm[i];
     h->st.ms_grid = ms_between(h->ev[0], h->ev[1]);
-    h->st.ms_count = ms_between(h->ev[1], h->ev[2]);
-    h->st.ms_scan = ms_between(h->ev[2], h->ev[3]);
+    h->st.ms_estimate = ms_between(h->ev[1], h->ev[2]);
+    h->st.ms_count = ms_between(h->ev[2], h->ev[3]);
+    h->st.ms_scan = ms_between(h->ev[3], h->ev[4]);
+    if (slotted)
+        GDSJ_TRY(estimator_error(h->G, h->estPred.as<float>(), d_counts, &h->st.est_mean_rel_err,
+                                 &h->st.est_max_rel_err, s, h->estScratch));
+    int rc = GDSJ_OK;
     if (!d_ids) {
-        h->st.ms_total = ms_between(h->ev[0], h->ev[3]);
-        return ids_cap == 0 && tot == 0 ? GDSJ_OK : (tot ? GDSJ_EOVERFLOW : GDSJ_OK);
-    }
-    if (tot > ids_cap) {
+        rc = tot ? GDSJ_EOVERFLOW : GDSJ_OK;
+    } else if (tot > ids_cap) {
         set_error("result has %llu pairs, output buffer holds %llu", (unsigned long long)tot,
                   (unsigned long long)ids_cap);
-        h->st.ms_total = ms_between(h->ev[0], h->ev[3]);
-        return GDSJ_EOVERFLOW;
+        rc = GDSJ_EOVERFLOW;
+    } else {
+        GDSJ_TRY(do_fill(h, d_ids, ids_cap));
     }
-    GDSJ_TRY(do_fill(h, d_ids, ids_cap));
-    h->st.ms_total = ms_between(h->ev[0], h->ev[6]);
-    return GDSJ_OK;
+    // ---- the estimator learns from the counts this join produced
+    const bool train = h->p.estimator == GDSJ_EST_LEARNED &&
+                       (h->p.train == 1 || (h->p.train == 0 && !h->est.trained));
+    if (train && h->G.haveRuns) {
+        double tms = 0;
+        GDSJ_TRY(estimator_train(h->est, h->G, d_counts, s, &tms));
+        h->st.ms_train = tms;
+    }
+    h->st.ms_total = h->st.ms_grid + h->st.ms_estimate + h->st.ms_count + h->st.ms_scan + h->st.ms_compact +
+                     h->st.ms_fill + h->st.ms_sort;
+    return rc;
 }
 
 int gdsj_fill_device(gdsj_handle *h, uint32_t *d_ids, uint64_t ids_cap) {
@@ -321,7 +429,8 @@ int gdsj_fill_device(gdsj_handle *h, uint32_t *d_ids, uint64_t ids_cap) {
     }
     GDSJ_HIP(hipSetDevice(h->device));
     GDSJ_TRY(do_fill(h, d_ids, ids_cap));
-    h->st.ms_total = h->st.ms_grid + h->st.ms_count + h->st.ms_scan + h->st.ms_fill + h->st.ms_sort;
+    h->st.ms_total = h->st.ms_grid + h->st.ms_estimate + h->st.ms_count + h->st.ms_scan + h->st.ms_compact +
+                     h->st.ms_fill + h->st.ms_sort;
     return GDSJ_OK;
 }
 
@@ -364,6 +473,81 @@ int gdsj_result_copy(gdsj_handle *h, uint32_t *ids, uint64_t cap) {
 int gdsj_get_stats(gdsj_handle *h, gdsj_stats *out) {
     if (!h || !out) return GDSJ_EINVAL;
     *out = h->st;
+    return GDSJ_OK;
+}
+
+// ------------------------------------------------------- learned estimator --
+uint32_t gdsj_mlp_param_count(void) { return (uint32_t)MLP_P; }
+uint32_t gdsj_mlp_feature_count(void) { return (uint32_t)MLP_F; }
+
+// host-buffer wrappers of the MLP kernels (used to check them against a CPU
+// restatement); run on the current device
+static int mlp_host_call(const float *params, const float *x, const float *t, uint32_t n, float *y, float *grad,
+                         float *loss) {
+    if (!params || (!x && n)) return GDSJ_EINVAL;
+    hipStream_t s = nullptr;
+    DevBuf dp, dx, dt, dy, dpart;
+    GDSJ_TRY(dp.ensure(MLP_P * 4));
+    GDSJ_TRY(dx.ensure((size_t)n * MLP_F * 4 + 64));
+    GDSJ_HIP(hipMemcpy(dp.p, params, MLP_P * 4, hipMemcpyHostToDevice));
+    if (n) GDSJ_HIP(hipMemcpy(dx.p, x, (size_t)n * MLP_F * 4, hipMemcpyHostToDevice));
+    int rc = GDSJ_OK;
+    if (y) {
+        rc = dy.ensure((size_t)n * 4 + 64);
+        if (!rc) rc = mlp_forward(dp.as<float>(), dx.as<float>(), n, dy.as<float>(), s);
+        if (!rc && n) rc = hipMemcpy(y, dy.p, (size_t)n * 4, hipMemcpyDeviceToHost) == hipSuccess ? 0 : GDSJ_EDEVICE;
+    } else {
+        rc = dt.ensure((size_t)n * 4 + 64);
+        if (!rc) rc = dpart.ensure(mlp_grad_scratch_floats(n) * 4 + (MLP_P + 8) * 4);
+        if (!rc && n) rc = hipMemcpy(dt.p, t, (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess ? 0 : GDSJ_EDEVICE;
+        float *g = dpart.as<float>() + mlp_grad_scratch_floats(n);
+        if (!rc) rc = mlp_grad(dp.as<float>(), dx.as<float>(), dt.as<float>(), n, g, nullptr, dpart.as<float>(), s);
+        std::vector<float> hg(MLP_P + 1);
+        if (!rc) rc = hipMemcpy(hg.data(), g, (MLP_P + 1) * 4, hipMemcpyDeviceToHost) == hipSuccess ? 0 : GDSJ_EDEVICE;
+        if (!rc) {
+            memcpy(grad, hg.data(), MLP_P * 4);
+            if (loss) *loss = hg[MLP_P];
+        }
+    }
+    DevBuf *b[] = {&dp, &dx, &dt, &dy, &dpart};
+    for (DevBuf *q : b) q->release();
+    return rc;
+}
+
+int gdsj_mlp_forward(const float *params, const float *x, uint32_t n, float *y) {
+    if (!y && n) return GDSJ_EINVAL;
+    return mlp_host_call(params, x, nullptr, n, y ? y : (float *)params, nullptr, nullptr);
+}
+
+int gdsj_mlp_grad(const float *params, const float *x, const float *t, uint32_t n, float *grad, float *loss) {
+    if (!grad || (!t && n) || n == 0) return GDSJ_EINVAL;
+    return mlp_host_call(params, x, t, n, nullptr, grad, loss);
+}
+
+int gdsj_model_get(gdsj_handle *h, float *params, uint32_t cap, double *alpha) {
+    if (!h || !params || cap < (uint32_t)MLP_P) return GDSJ_EINVAL;
+    if (!h->est.init) {
+        set_error("the estimator has not been initialised (no learned join yet)");
+        return GDSJ_ESTATE;
+    }
+    GDSJ_HIP(hipSetDevice(h->device));
+    GDSJ_HIP(hipMemcpy(params, h->est.params.p, MLP_P * 4, hipMemcpyDeviceToHost));
+    if (alpha) *alpha = h->est.alpha;
+    return GDSJ_OK;
+}
+
+int gdsj_model_set(gdsj_handle *h, const float *params, uint32_t count, double alpha) {
+    if (!h || !params || count != (uint32_t)MLP_P || !(alpha >= 0)) return GDSJ_EINVAL;
+    GDSJ_HIP(hipSetDevice(h->device));
+    GDSJ_TRY(h->est.params.ensure(MLP_P * 4));
+    GDSJ_TRY(h->est.adam_m.ensure(MLP_P * 4));
+    GDSJ_TRY(h->est.adam_v.ensure(MLP_P * 4));
+    GDSJ_HIP(hipMemcpy(h->est.params.p, params, MLP_P * 4, hipMemcpyHostToDevice));
+    GDSJ_HIP(hipMemset(h->est.adam_m.p, 0, MLP_P * 4));
+    GDSJ_HIP(hipMemset(h->est.adam_v.p, 0, MLP_P * 4));
+    h->est.alpha = alpha;
+    h->est.init = h->est.trained = true;
+    h->est.step = 0;
     return GDSJ_OK;
 }
 

@@ -20,6 +20,7 @@ namespace gdsj {
 
 constexpr int MAXK = 12;
 constexpr int MAXD = 1024;
+constexpr int RUN_HIST_BINS = 6;  // neighbor-run length histogram per cell
 
 // what the join kernels need to enumerate adjacent cells
 struct GridArgs {
@@ -47,7 +48,7 @@ struct Grid {
     uint64_t ld = 0;  // leading dimension of xs
     // device buffers
     DevBuf xs, ids, cellKey, cellStart, cellCoord, hkeys, hvals, tiles;
-    DevBuf runs, nruns, cellCand;  // per-cell neighbor runs (build_runs)
+    DevBuf runs, nruns, cellCand, cellHist;  // per-cell neighbor runs (build_runs)
     uint64_t runStride = 0;
     bool haveRuns = false;
     DevBuf keyA, keyB, valB, tmp, aux, flag, tileKey, tileVal, cellAux, scalars;

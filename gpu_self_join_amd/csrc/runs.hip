@@ -20,7 +20,8 @@ __global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uin
                                                     const uint64_t *__restrict__ hkeys,
                                                     const uint32_t *__restrict__ hvals, uint32_t ncombo,
                                                     uint64_t stride, int prune, uint2 *__restrict__ runs,
-                                                    uint32_t *__restrict__ nruns, uint32_t *__restrict__ cellCand) {
+                                                    uint32_t *__restrict__ nruns, uint32_t *__restrict__ cellCand,
+                                                    float *__restrict__ cellHist) {
     const uint32_t lane = lane_id();
     const uint32_t cell = uniform((uint32_t)(blockIdx.x * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
     if (cell >= ga.ncell) return;
@@ -41,6 +42,7 @@ __global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uin
     }
     uint32_t cursor = 0;
     uint32_t cand = 0;
+    uint32_t hist[RUN_HIST_BINS] = {};
     uint2 *mine = runs + (uint64_t)cell * stride;
     for (uint32_t cw = 0; cw < ncombo; cw += GDSJ_WAVE) {
         uint2 run = make_uint2(0u, 0u);
@@ -50,6 +52,10 @@ __global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uin
         if (ne) mine[cursor + mbcnt(m)] = run;
         cursor += popc64(m);
         uint32_t len = run.y - run.x;
+        // run-length histogram (estimator features): 1, 2-3, 4-7, 8-15, 16-63, 64+
+        const uint32_t bin = len < 2 ? 0u : len < 4 ? 1u : len < 8 ? 2u : len < 16 ? 3u : len < 64 ? 4u : 5u;
+#pragma unroll
+        for (int b = 0; b < RUN_HIST_BINS; b++) hist[b] += popc64(ballot(ne && bin == (uint32_t)b));
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) len += __shfl_xor(len, off, GDSJ_WAVE);
         cand += uniform(len);
@@ -57,6 +63,9 @@ __global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uin
     if (lane == 0) {
         nruns[cell] = cursor;
         cellCand[cell] = cand;
+        const float inv = cursor ? 1.0f / (float)cursor : 0.0f;
+#pragma unroll
+        for (int b = 0; b < RUN_HIST_BINS; b++) cellHist[(uint64_t)cell * RUN_HIST_BINS + b] = (float)hist[b] * inv;
     }
 }
 
@@ -70,19 +79,20 @@ int build_runs(Grid &G, hipStream_t s, bool prune, bool *built) {
     GDSJ_TRY(G.runs.ensure(entries * 8 + 64));
     GDSJ_TRY(G.nruns.ensure((size_t)G.ncell * 4 + 64));
     GDSJ_TRY(G.cellCand.ensure((size_t)G.ncell * 4 + 64));
+    GDSJ_TRY(G.cellHist.ensure((size_t)G.ncell * RUN_HIST_BINS * 4 + 64));
     const uint32_t blocks = (G.ncell + 3) / 4;
     if (G.dtype == GDSJ_DTYPE_F32)
         k_cell_runs<float><<<blocks, 256, 0, s>>>(G.xs.as<float>(), G.ld, G.ga, G.cellStart.as<uint32_t>(),
                                                   G.cellCoord.as<uint32_t>(), G.hkeys.as<uint64_t>(),
                                                   G.hvals.as<uint32_t>(), (uint32_t)ncombo, ncombo, prune ? 1 : 0,
                                                   G.runs.as<uint2>(), G.nruns.as<uint32_t>(),
-                                                  G.cellCand.as<uint32_t>());
+                                                  G.cellCand.as<uint32_t>(), G.cellHist.as<float>());
     else
         k_cell_runs<double><<<blocks, 256, 0, s>>>(G.xs.as<double>(), G.ld, G.ga, G.cellStart.as<uint32_t>(),
                                                    G.cellCoord.as<uint32_t>(), G.hkeys.as<uint64_t>(),
                                                    G.hvals.as<uint32_t>(), (uint32_t)ncombo, ncombo,
                                                    prune ? 1 : 0, G.runs.as<uint2>(), G.nruns.as<uint32_t>(),
-                                                   G.cellCand.as<uint32_t>());
+                                                   G.cellCand.as<uint32_t>(), G.cellHist.as<float>());
     GDSJ_HIP(hipGetLastError());
     *built = true;
     return GDSJ_OK;

@@ -69,7 +69,9 @@ typedef struct gdsj_params {
     uint32_t schedule;      /* tile->workgroup order: 0 auto, 1 round-robin, 2 XCD-contiguous */
     uint32_t prune;         /* skip neighbor cells farther than eps from a tile's query box:
                                0 auto, 1 on, 2 off (speed only) */
-    uint32_t reserved[3];
+    uint32_t train;         /* learned estimator: 0 train until first trained, 1 after every
+                               join, 2 never */
+    uint32_t reserved[2];
 } gdsj_params;
 
 /* Timing and diagnostics of the last join on a handle. */
@@ -80,16 +82,24 @@ typedef struct gdsj_stats {
     uint64_t total_cells;        /* product of cells per indexed dim */
     uint64_t n_tiles;
     uint64_t n_runs;
-    uint64_t filter_rechecks;    /* Gram filter: pairs sent to exact recheck */
-    uint64_t filter_band;        /* Gram filter: rechecks that were not hits */
+    uint64_t filter_rechecks;    /* Gram filter: pairs sent to the exact recheck */
+    uint64_t filter_hits;        /* Gram filter: rechecked pairs that are neighbors */
     uint64_t est_predicted;      /* estimator: predicted total pairs */
-    uint64_t est_overflow_tiles; /* learned estimator: tiles re-run exactly */
-    double ms_grid;              /* reorder + grid + runs + tiles (device) */
-    double ms_count;             /* counting kernel */
+    uint64_t est_capacity;       /* estimator: result slots allocated (pairs) */
+    uint64_t est_overflow_tiles; /* tiles whose slot overflowed (completed exactly) */
+    uint64_t est_tiles;          /* tiles sized by the estimator */
+    double est_mean_rel_err;     /* mean |predicted - actual| / actual over tiles */
+    double est_max_rel_err;
+    double filter_max_err;       /* reserved */
+    double ms_grid;              /* reorder + grid + neighbor runs (device) */
+    double ms_estimate;          /* estimator inference + slot sizing */
+    double ms_count;             /* counting (or slotted) join pass */
     double ms_scan;              /* exclusive scan of counts -> CSR offsets */
-    double ms_fill;              /* fill kernel */
+    double ms_compact;           /* slot compaction into the CSR */
+    double ms_fill;              /* fill pass (all tiles, or overflowed tiles) */
     double ms_sort;              /* per-point row sort (sort_neighbors) */
-    double ms_total;             /* whole join, first to last device event */
+    double ms_train;             /* estimator training on this join's counts */
+    double ms_total;             /* sum of the phases above except training */
     uint32_t perm[64];           /* dimension order after REORDER (first 64) */
 } gdsj_stats;
 
@@ -133,6 +143,19 @@ int gdsj_get_stats(gdsj_handle *h, gdsj_stats *out);
  * reference's per-pair square root (kernel.cu:361,364). */
 float gdsj_eps_threshold_f32(float eps);
 double gdsj_eps_threshold_f64(double eps);
+
+/* ---- learned estimator ------------------------------------------------- */
+/* The 16-32-32-1 MLP that sizes per-tile result slots (no counterpart in the
+ * reference, which samples every 66th query: GPU.cu:57-268).  Its weights
+ * can be saved and restored; the two host-buffer calls run its forward and
+ * gradient kernels on the current device. */
+uint32_t gdsj_mlp_param_count(void);
+uint32_t gdsj_mlp_feature_count(void);
+int gdsj_mlp_forward(const float *params, const float *x, uint32_t n, float *y);
+/* grad = d/dparams mean_i (MLP(x_i) - t_i)^2, *loss = that mean */
+int gdsj_mlp_grad(const float *params, const float *x, const float *t, uint32_t n, float *grad, float *loss);
+int gdsj_model_get(gdsj_handle *h, float *params, uint32_t cap, double *alpha);
+int gdsj_model_set(gdsj_handle *h, const float *params, uint32_t count, double alpha);
 
 /* Thread-local text of the last error. */
 const char *gdsj_last_error(void);
