@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "estimator.h"
+#include "gram.h"
 #include "grid.h"
 #include "join.h"
 #include "scan.h"
@@ -47,6 +48,9 @@ struct gdsj_handle {
     // slotted single pass (estimator-sized per-tile result slots)
     DevBuf slotCap, slotOff, slotUsed, slotOvf, slotIds, slotQ, scalars, estPred, estScratch;
     Estimator est;
+    GramData gram;
+    DevBuf gramStats;
+    bool use_gram = false;
     bool slotted = false;
     uint32_t novf = 0;
     gdsj_params p{};
@@ -76,7 +80,7 @@ int gdsj_params_default(gdsj_params *p) {
     p->reorder = 1;
     p->stamp = 0;
     p->queryreorder = 1;
-    p->filter = GDSJ_FILTER_PLAIN;
+    p->filter = GDSJ_FILTER_AUTO;
     p->estimator = GDSJ_EST_EXACT;
     p->sort_neighbors = 0;
     p->verbose = 0;
@@ -145,6 +149,8 @@ int gdsj_destroy(gdsj_handle *h) {
                    &h->work,    &h->slotCap,  &h->slotOff, &h->slotUsed, &h->slotOvf, &h->slotIds,
                    &h->slotQ,   &h->scalars,  &h->estPred, &h->estScratch};
     h->est.release();
+    h->gram.release();
+    h->gramStats.release();
     for (DevBuf *x : b) x->release();
     for (int i = 0; i < 8; i++)
         if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
@@ -207,6 +213,14 @@ static JoinArgs make_args(gdsj_handle *h) {
 // Writes the table of the last join into d_ids: the exact fill pass, or,
 // after a slotted pass, compaction of the slots plus the fill pass for the
 // tiles whose slot overflowed.
+// one join pass through the plain kernels or the MFMA Gram filter
+static int run_pass(gdsj_handle *h, const JoinArgs &a, int pass) {
+    if (h->use_gram)
+        return gram_join_pass(h->G, h->gram, a, h->mode, pass,
+                              pass == PASS_FILL ? nullptr : h->gramStats.as<unsigned long long>(), h->stream);
+    return join_pass(h->G, a, h->mode, pass, h->stream);
+}
+
 static void set_slot_args(gdsj_handle *h, JoinArgs &a) {
     a.slotOff = h->slotOff.as<uint64_t>();
     a.slotCap = h->slotCap.as<uint32_t>();
@@ -232,10 +246,10 @@ static int do_fill(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
         if (h->novf) {
             a.tileList = h->slotOvf.as<uint32_t>();
             a.ntiles = h->novf;
-            GDSJ_TRY(join_pass(h->G, a, h->mode, PASS_FILL, s));
+            GDSJ_TRY(run_pass(h, a, PASS_FILL));
         }
     } else {
-        GDSJ_TRY(join_pass(h->G, a, h->mode, PASS_FILL, s));
+        GDSJ_TRY(run_pass(h, a, PASS_FILL));
         GDSJ_HIP(hipEventRecord(h->ev[7], s));
     }
     GDSJ_HIP(hipEventRecord(h->ev[5], s));
@@ -299,7 +313,7 @@ static int slotted_pass(gdsj_handle *h, JoinArgs &a, bool *used) {
     GDSJ_HIP(hipMemsetAsync(h->scalars.p, 0, 4, s));
     GDSJ_HIP(hipEventRecord(h->ev[2], s));
     set_slot_args(h, a);
-    GDSJ_TRY(join_pass(G, a, h->mode, PASS_SLOT, s));
+    GDSJ_TRY(run_pass(h, a, PASS_SLOT));
     *used = true;
     return GDSJ_OK;
 }
@@ -334,8 +348,20 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     memset(&h->st, 0, sizeof(h->st));
     GDSJ_TRY(h->status.ensure(256));
     GDSJ_HIP(hipEventRecord(h->ev[0], s));
+    // the MFMA Gram filter for dims >= 16 (AUTO) or on request
+    h->use_gram = (h->p.filter == GDSJ_FILTER_GRAM || h->p.filter == GDSJ_FILTER_AUTO) && gram_supported(dims);
+    if (h->p.filter == GDSJ_FILTER_GRAM && !h->use_gram) {
+        set_error("the Gram filter handles 16 to 128 dims (dims=%u)", dims);
+        return GDSJ_EUNSUPPORTED;
+    }
     uint32_t tq = h->p.tile_queries ? h->p.tile_queries : 64;
+    if (h->use_gram && tq > 32) tq = 32;
     GDSJ_TRY(grid_build(h->G, d_data, n, dims, dtype, epsilon, h->p.k, h->p.reorder, tq, h->p.queryreorder, s));
+    if (h->use_gram) {
+        GDSJ_TRY(gram_prepare(h->gram, h->G, dtype == GDSJ_DTYPE_F64 ? 1 : 0, s));
+        GDSJ_TRY(h->gramStats.ensure(64));
+        GDSJ_HIP(hipMemsetAsync(h->gramStats.p, 0, 16, s));
+    }
     {
         bool built = false;
         const bool prune = h->p.prune == 1 || (h->p.prune == 0 && h->G.k >= 4);
@@ -361,7 +387,7 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     if (!slotted) {
         GDSJ_HIP(hipEventRecord(h->ev[1], s));
         GDSJ_HIP(hipEventRecord(h->ev[2], s));
-        GDSJ_TRY(join_pass(h->G, a, h->mode, PASS_COUNT, s));
+        GDSJ_TRY(run_pass(h, a, PASS_COUNT));
     }
     h->slotted = slotted;
     GDSJ_HIP(hipEventRecord(h->ev[3], s));
@@ -369,7 +395,8 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     GDSJ_HIP(hipEventRecord(h->ev[4], s));
     uint64_t *candSum = h->status.as<uint64_t>() + 4;
     GDSJ_TRY(sum_u64(a.tileWork, h->G.ntiles, candSum, s));
-    uint64_t hres[3] = {0, 0, 0};
+    uint64_t hres[3] = {0, 0, 0}, gst[2] = {0, 0};
+    if (h->use_gram) GDSJ_HIP(hipMemcpyAsync(gst, h->gramStats.p, 16, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipMemcpyAsync(&hres[0], offs + n, 8, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipMemcpyAsync(&hres[1], candSum, 8, hipMemcpyDeviceToHost, s));
     if (slotted) GDSJ_HIP(hipMemcpyAsync(&hres[2], h->scalars.p, 4, hipMemcpyDeviceToHost, s));
@@ -386,6 +413,8 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     h->st.n_tiles = h->G.ntiles;
     h->st.n_runs = h->G.haveRuns ? 0 : (uint64_t)h->G.ntiles * a.ncombo;
     h->st.est_overflow_tiles = h->novf;
+    h->st.filter_rechecks = gst[0];
+    h->st.filter_hits = gst[1];
     for (uint32_t i = 0; i < dims && i < 64; i++) h->st.perm[i] = h->G.perm[i];
     h->st.ms_grid = ms_between(h->ev[0], h->ev[1]);
     h->st.ms_estimate = ms_between(h->ev[1], h->ev[2]);

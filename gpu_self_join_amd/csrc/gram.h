@@ -1,0 +1,39 @@
+// gram.h -- MFMA Gram-form distance filter for dims >= 16.
+//
+// Every point x (dims permuted, sorted order) is stored once per join as a
+// low-precision vector h = lp(sigma (x - o)) (bf16 for FP32 data, fp16 for
+// FP64 data; o = per-dim mean, sigma a power of two keeping fp16 in range)
+// plus one float W = (1-c)/2 |h|^2 - (eps'/2 + e)^2, where e bounds |x' - h|
+// (measured exactly per point) and c bounds the FP32 accumulation error of
+// the MFMA dot products.  For a query q and candidate p the 32x32x16 MFMA
+// gives S ~ h_q . h_p, and
+//     S < W_q + W_p   =>   |x_q - x_p| > eps'   (no pair possible)
+// so only pairs with S >= W_q + W_p are rechecked by direct differencing in
+// DTYPE with the reference's accumulation order (kernel.cu:307-365).  The
+// filter never decides a pair: the result equals the plain join's.
+#pragma once
+#include "common.h"
+#include "grid.h"
+#include "join.h"
+
+namespace gdsj {
+
+struct GramData {
+    DevBuf h;        // [n][dpad] bf16 or fp16
+    DevBuf w;        // [n] float
+    DevBuf scratch;  // per-dim sums / maxima, stats
+    uint32_t dpad = 0;
+    int lowp = 0;    // 0 bf16, 1 fp16
+    void release();
+};
+
+// dims the filter handles (padded to a multiple of 16)
+bool gram_supported(uint32_t d);
+// Builds h and w for the grid's points; lowp: 0 bf16, 1 fp16.
+int gram_prepare(GramData &gd, const Grid &G, int lowp, hipStream_t s);
+// One pass (JoinPass) of the filtered join (tiles of <= 32 queries).
+// stats[0] += rechecked pairs, stats[1] += rechecked pairs that are hits.
+int gram_join_pass(const Grid &G, const GramData &gd, const JoinArgs &a, int mode, int pass,
+                   unsigned long long *stats, hipStream_t s);
+
+}  // namespace gdsj

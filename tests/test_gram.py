@@ -1,0 +1,98 @@
+"""The MFMA Gram-form filter (dims >= 16): identical tables to the plain join
+and to the oracle, including pairs at exactly eps, and filter statistics."""
+import numpy as np
+import pytest
+
+from gpu_self_join_amd import _lib
+from oracle import oracle as O
+from tests.helpers import assert_csr_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _join(sj, x, eps, **kw):
+    import torch
+    r = sj.join(torch.from_numpy(np.ascontiguousarray(x)).cuda(), eps, **kw)
+    return r.counts.cpu().numpy().view(np.uint32), r.ids.cpu().numpy().view(np.uint32), r
+
+
+def _clustered(n, d, dt, seed, spread=1.0):
+    rng = np.random.default_rng(seed)
+    c = rng.random((6, d)) * 30
+    return (c[rng.integers(0, 6, n)] + spread * rng.standard_normal((n, d))).astype(dt)
+
+
+def _eps(x, nb):
+    rng = np.random.default_rng(1)
+    q = x[rng.choice(len(x), 64, replace=False)].astype(np.float64)
+    dd = np.sqrt(((q[:, None, :] - x[None, :, :].astype(np.float64)) ** 2).sum(-1))
+    return float(np.asarray(np.sort(dd, axis=1)[:, nb].mean(), dtype=x.dtype))
+
+
+CASES = [(3000, 16, 6, np.float32, 8, 1, 1), (3000, 32, 6, np.float32, 8, 1, 1), (2000, 64, 8, np.float32, 0, 0, 1),
+         (2500, 20, 4, np.float32, 8, 0, 0), (2000, 100, 6, np.float32, 8, 1, 1),
+         (2000, 16, 6, np.float64, 8, 1, 1), (2000, 64, 8, np.float64, 8, 1, 1), (1500, 40, 5, np.float64, 0, 0, 0)]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "n%d_d%d_k%d_%s_ilp%d_sc%d_r%d" % (
+    c[0], c[1], c[2], np.dtype(c[3]).name, c[4], c[5], c[6]))
+def test_gram_equals_plain_and_oracle(sj, case):
+    n, d, k, dt, ilp, sc, reorder = case
+    x = _clustered(n, d, dt, n + d)
+    eps = _eps(x, 25)
+    kw = dict(k=k, ilp=ilp, shortcircuit=sc, reorder=reorder)
+    cg, ig, rg = _join(sj, x, eps, filter=_lib.FILTER_GRAM, **kw)
+    cp, ip, rp = _join(sj, x, eps, filter=_lib.FILTER_PLAIN, **kw)
+    print("case", case, "total", rg.total, "rechecks", rg.stats["filter_rechecks"], "hits",
+          rg.stats["filter_hits"], flush=True)
+    assert np.array_equal(cg, cp) and np.array_equal(ig, ip)  # same rows, same order
+    assert rg.stats["filter_hits"] == rg.total
+    assert rg.stats["filter_rechecks"] >= rg.total
+    ref = O.join(x, eps, k=k, ilp=ilp, shortcircuit=sc, reorder=reorder)
+    assert_csr_equal(cg, ig, ref["counts"], ref["ids"])
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_gram_pairs_at_exactly_eps(sj, dt):
+    # points on a 16-D lattice of spacing eps: neighbors at distance exactly
+    # eps, sqrt(2) eps, ... -- the filter must send every boundary pair to the
+    # exact recheck
+    side = 3
+    g = np.stack(np.meshgrid(*[np.arange(side)] * 4, indexing="ij"), -1).reshape(-1, 4)
+    rng = np.random.default_rng(3)
+    basis = np.zeros((4, 16))
+    basis[:, :4] = np.eye(4)
+    eps = 0.75
+    x = (g @ basis * eps + rng.random(16) * 50).astype(dt)
+    x = np.concatenate([x, x + dt(0.01 * eps)], axis=0)
+    e = float(np.asarray(eps, dtype=dt))
+    for ilp in (0, 8):
+        cg, ig, rg = _join(sj, x, e, k=4, ilp=ilp, shortcircuit=1, reorder=0, filter=_lib.FILTER_GRAM)
+        ref = O.join(x, e, k=4, ilp=ilp, shortcircuit=1, reorder=0)
+        assert_csr_equal(cg, ig, ref["counts"], ref["ids"])
+
+
+def test_gram_large_coordinates(sj):
+    # far from the origin: the Gram form cancels catastrophically unless the
+    # bound accounts for it; the result must still be exact
+    x = _clustered(2000, 24, np.float32, 5) + np.float32(3.0e4)
+    eps = _eps(x, 20)
+    cg, ig, rg = _join(sj, x, eps, k=6, ilp=8, shortcircuit=1, reorder=1, filter=_lib.FILTER_GRAM)
+    ref = O.join(x, eps, k=6, ilp=8, shortcircuit=1, reorder=1)
+    assert_csr_equal(cg, ig, ref["counts"], ref["ids"])
+
+
+def test_gram_with_learned_slots(sj):
+    from gpu_self_join_amd import SelfJoin
+    x = _clustered(20000, 32, np.float32, 9, spread=2.0)
+    eps = _eps(x, 30)
+    kw = dict(k=6, ilp=8, shortcircuit=1, reorder=1, filter=_lib.FILTER_GRAM, estimator=_lib.EST_LEARNED)
+    h = SelfJoin(0)
+    try:
+        _join(h, x, eps, **kw)
+        c, ids, r = _join(h, x, eps, **kw)
+        assert r.stats["est_tiles"] > 0
+    finally:
+        h.close()
+    ref = O.join(x, eps, k=6, ilp=8, shortcircuit=1, reorder=1)
+    assert_csr_equal(c, ids, ref["counts"], ref["ids"])
