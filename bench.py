@@ -161,6 +161,8 @@ def main():
         st = vr["stats"]
         variants[vname] = {"ms_per_step": round(vr["ms_per_step"], 4), "pairs_per_s": vr["pairs_per_s"],
                            "phase_ms": {k: round(v, 4) for k, v in vr["phase_ms"].items()}}
+        variants[vname]["filter"] = {"rechecks": st["filter_rechecks"], "hits": st["filter_hits"],
+                                     "candidates": st["candidates"]}
         if vname != "exact":
             variants[vname]["estimator"] = {k: st[k] for k in ("est_predicted", "est_capacity", "est_overflow_tiles",
                                                                 "est_tiles", "est_mean_rel_err", "est_max_rel_err")}

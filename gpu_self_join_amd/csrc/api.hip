@@ -355,7 +355,12 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
         return GDSJ_EUNSUPPORTED;
     }
     uint32_t tq = h->p.tile_queries ? h->p.tile_queries : 64;
-    if (h->use_gram && tq > 32) tq = 32;
+    if (h->use_gram) {
+        const uint32_t gq = gram_tile_queries(dims, dtype == GDSJ_DTYPE_F64 ? 1 : 0);
+        if (!h->p.tile_queries || tq > gq) tq = gq;
+    } else if (tq > 64) {
+        tq = 64;  // the plain kernels keep one query per lane
+    }
     GDSJ_TRY(grid_build(h->G, d_data, n, dims, dtype, epsilon, h->p.k, h->p.reorder, tq, h->p.queryreorder, s));
     if (h->use_gram) {
         GDSJ_TRY(gram_prepare(h->gram, h->G, dtype == GDSJ_DTYPE_F64 ? 1 : 0, s));

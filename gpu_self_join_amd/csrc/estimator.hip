@@ -333,21 +333,27 @@ int estimator_train(Estimator &E, const Grid &G, const uint32_t *counts, hipStre
         E.init = true;
         E.step = 0;
     }
-    const uint32_t batch = 2048;
-    const int epochs = E.trained ? 2 : 8;
+    // minibatch Adam: a fixed number of steps (first training: 600, later
+    // joins: 150), cycling over the sample in a shuffled order per epoch
+    const uint32_t batch = std::min<uint32_t>(2048, m);
+    const uint32_t steps = E.trained ? 150 : 600;
     GDSJ_TRY(E.partial.ensure(mlp_grad_scratch_floats(batch) * 4 + (MLP_P + 8) * 4));
     float *partial = E.partial.as<float>();
     float *grad = partial + mlp_grad_scratch_floats(batch);
     const uint32_t pa = coprime_step(m);
-    for (int ep = 0; ep < epochs; ep++) {
-        const uint32_t pc = (uint32_t)((ep * 2654435761ull) % m);
-        for (uint32_t b = 0; b < m; b += batch) {
-            const uint32_t cnt = std::min(batch, m - b);
-            GDSJ_TRY(mlp_grad_rows(E.params.as<float>(), sx, st, m, b, cnt, pa, pc, grad, partial, s));
-            E.step++;
-            GDSJ_TRY(mlp_adam(E.params.as<float>(), grad, E.adam_m.as<float>(), E.adam_v.as<float>(), E.step,
-                              3e-3f, s));
+    uint32_t b = 0, ep = 0;
+    for (uint32_t it = 0; it < steps; it++) {
+        if (b >= m) {
+            b = 0;
+            ep++;
         }
+        const uint32_t pc = (uint32_t)((ep * 2654435761ull) % m);
+        const uint32_t cnt = std::min(batch, m - b);
+        GDSJ_TRY(mlp_grad_rows(E.params.as<float>(), sx, st, m, b, cnt, pa, pc, grad, partial, s));
+        E.step++;
+        const float lr = it < steps / 2 ? 3e-3f : 1e-3f;
+        GDSJ_TRY(mlp_adam(E.params.as<float>(), grad, E.adam_m.as<float>(), E.adam_v.as<float>(), E.step, lr, s));
+        b += cnt;
     }
     // slot headroom: 99th percentile of actual / predicted hits on the sample
     GDSJ_TRY(E.pred.ensure((size_t)std::max(nt, m) * 4));

@@ -19,16 +19,20 @@
 namespace gdsj {
 
 struct GramData {
-    DevBuf h;        // [n][dpad] bf16 or fp16
+    DevBuf h;        // [n][dpad] bf16 or fp16: leading part of sigma (x - o)
+    DevBuf l;        // [n][dpad] the remainder in the same format (split form only)
     DevBuf w;        // [n] float
     DevBuf scratch;  // per-dim sums / maxima, stats
     uint32_t dpad = 0;
     int lowp = 0;    // 0 bf16, 1 fp16
+    int split = 0;   // 1: S = h.h' + h.l' + l.h' (three MFMAs per k-step)
     void release();
 };
 
 // dims the filter handles (padded to a multiple of 16)
 bool gram_supported(uint32_t d);
+// queries per Gram tile (a multiple of 32) for dims d and operand format lowp
+uint32_t gram_tile_queries(uint32_t d, int lowp);
 // Builds h and w for the grid's points; lowp: 0 bf16, 1 fp16.
 int gram_prepare(GramData &gd, const Grid &G, int lowp, hipStream_t s);
 // One pass (JoinPass) of the filtered join (tiles of <= 32 queries).

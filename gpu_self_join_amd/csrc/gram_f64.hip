@@ -1,0 +1,11 @@
+// gram_f64.hip -- Gram-filter kernels for FP64 data (fp16 operands).
+#include "gram_kernels.h"
+
+namespace gdsj {
+
+int gram_dispatch_f64(const GramData &gd, const JoinArgs &a, int mode, int pass, unsigned long long *stats,
+                      hipStream_t s) {
+    return gram_dispatch<double, 1>(gd, a, mode, pass, stats, s);
+}
+
+}  // namespace gdsj

@@ -1,0 +1,316 @@
+// gram_kernels.h -- the Gram-filter join kernel template (instantiated per
+// DTYPE in gram_f32.hip / gram_f64.hip); see gram.hip for the algorithm.
+#pragma once
+#include "gram.h"
+#include "join_kernels.h"
+#include "neighbors.h"
+
+namespace gdsj {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <int LP> struct Lp;
+template <> struct Lp<0> {
+    using scalar = __bf16;
+    using vec = bf16x8;
+    static constexpr double min_normal = 0x1p-126;
+};
+template <> struct Lp<1> {
+    using scalar = _Float16;
+    using vec = f16x8;
+    static constexpr double min_normal = 0x1p-14;
+};
+
+template <int LP>
+__device__ __forceinline__ f32x16 mfma32(typename Lp<LP>::vec a, typename Lp<LP>::vec b, f32x16 c) {
+    if constexpr (LP == 0)
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+template <int LP>
+__device__ __forceinline__ typename Lp<LP>::vec load8(const typename Lp<LP>::scalar *p) {
+    const uint4 raw = *reinterpret_cast<const uint4 *>(p);
+    return __builtin_bit_cast(typename Lp<LP>::vec, raw);
+}
+
+// row blocks of 32 queries per wave: A operands of all blocks stay in VGPRs
+constexpr int rb_max(int ks, int split) {
+    const int r = 16 / (ks * (split ? 2 : 1));
+    return r < 1 ? 1 : (r > 4 ? 4 : r);
+}
+
+// ---------------------------------------------------------------- the join --
+template <typename T, int LP, int KS, int SPLIT, int PASS>
+__global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
+                                              const typename Lp<LP>::scalar *__restrict__ H,
+                                              const typename Lp<LP>::scalar *__restrict__ Lo,
+                                              const float *__restrict__ W, const JoinArgs a, int mode,
+                                              unsigned long long *__restrict__ stats) {
+    using vec = typename Lp<LP>::vec;
+    constexpr uint32_t DP = KS * 16;
+    constexpr int RB = rb_max(KS, SPLIT);
+    constexpr uint32_t TQ = 32 * RB;
+    const uint32_t lane = lane_id();
+    const uint32_t w = threadIdx.x / GDSJ_WAVE;
+    const uint32_t lblock = a.xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t tslot = uniform((uint32_t)(lblock * (blockDim.x / GDSJ_WAVE) + w));
+    if (tslot >= a.ntiles) return;
+    const uint32_t tile = a.tileList ? uniform(a.tileList[tslot]) : tslot;
+    const uint64_t ld = a.ld;
+    const T thr = (T)a.thr;
+    const uint4 tl = a.tiles[tile];
+    const uint32_t cell = uniform(tl.x), q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
+    const uint32_t nrb = (nq + 31) / 32;
+    const GridArgs &ga = a.ga;
+    const bool pre = a.runs != nullptr;
+    const uint32_t row_l = lane & 31u, half = lane >> 5;
+    const uint64_t ltm = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+
+    __shared__ double s_gap[256 / GDSJ_WAVE][2][MAXK];
+    __shared__ uint32_t s_cnt[256 / GDSJ_WAVE][TQ];
+    __shared__ unsigned long long s_cur[256 / GDSJ_WAVE][TQ];
+    __shared__ uint32_t s_hm[256 / GDSJ_WAVE][TQ];  // hit columns per row, current block
+    __shared__ uint32_t s_cid[256 / GDSJ_WAVE][32];  // ids of the current block's columns
+    double *glo = s_gap[w][0];
+    double *ghi = s_gap[w][1];
+    uint32_t cc[MAXK];
+    uint32_t nwin;
+    if (pre) {
+        nwin = uniform(a.nruns[cell]);
+    } else {
+        nwin = a.ncombo;
+        for (uint32_t j = 0; j < ga.k; j++) cc[j] = uniform(a.cellCoord[(uint64_t)j * ga.ncell + cell]);
+        if (a.prune) {
+            tile_gaps<T>(xs, ld, q0, nq, cc, ga, glo, ghi);
+        } else {
+            if (lane < MAXK) {
+                glo[lane] = 0.0;
+                ghi[lane] = 0.0;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    for (uint32_t r = lane; r < TQ; r += GDSJ_WAVE) {
+        s_cnt[w][r] = 0;
+        s_hm[w][r] = 0;
+        s_cur[w][r] = (PASS == PASS_FILL && r < nq) ? a.offsets[ids[q0 + r]] : 0ull;
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    vec A[RB][KS];
+    vec Al[RB][KS];
+    float wq[RB][16];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        const uint32_t qr = rb * 32 + row_l;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            if (qr < nq) {
+                A[rb][ks] = load8<LP>(H + (uint64_t)(q0 + qr) * DP + ks * 16 + half * 8);
+                if (SPLIT) Al[rb][ks] = load8<LP>(Lo + (uint64_t)(q0 + qr) * DP + ks * 16 + half * 8);
+            } else {
+                A[rb][ks] = (vec){};
+                if (SPLIT) Al[rb][ks] = (vec){};
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t r = rb * 32 + (i / 4) * 8 + half * 4 + (i % 4);
+            wq[rb][i] = r < nq ? W[q0 + r] : INFINITY;
+        }
+    }
+    uint32_t scur = 0, scap = 0;
+    uint64_t sbase = 0;
+    if (PASS == PASS_SLOT) {
+        scap = uniform(a.slotCap[tile]);
+        sbase = uniform64(a.slotOff[tile]);
+    }
+    uint64_t work = 0;
+    uint32_t nrech = 0, nhit = 0;
+    bool overflow = false;
+    static_assert(TQ <= 128, "rows are owned by lanes l and l + 64");
+
+    for (uint32_t cw = 0; cw < nwin; cw += GDSJ_WAVE) {
+        const uint32_t nrw = uniform(nwin - cw < (uint32_t)GDSJ_WAVE ? nwin - cw : (uint32_t)GDSJ_WAVE);
+        uint2 run = make_uint2(0u, 0u);
+        if (lane < nrw)
+            run = pre ? a.runs[(uint64_t)cell * a.runStride + cw + lane]
+                      : combo_run(cw + lane, cc, glo, ghi, ga, a.cellStart, a.hkeys, a.hvals);
+        const uint32_t len = run.y - run.x;
+        uint32_t incl = len;
+#pragma unroll
+        for (int off = 1; off < GDSJ_WAVE; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off, GDSJ_WAVE);
+            if (lane >= (uint32_t)off) incl += o;
+        }
+        const uint32_t total = uniform(shfl_u32(incl, nrw - 1));
+        if (total == 0) continue;
+        work += total;
+        const uint32_t rexcl = incl - len;
+        const uint32_t rstart = run.x;
+        const uint32_t fpos = uniform(cand_pos(0u, nrw, rstart, rexcl));
+        for (uint32_t cb = 0; cb < total; cb += 32) {
+            const uint32_t g = cb + row_l;
+            const bool valid = g < total;
+            uint32_t sp = cand_pos(valid ? g : 0u, nrw, rstart, rexcl);
+            sp = valid ? sp : fpos;
+            vec B[KS], Bl[KS];
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                B[ks] = load8<LP>(H + (uint64_t)sp * DP + ks * 16 + half * 8);
+                if (SPLIT) Bl[ks] = load8<LP>(Lo + (uint64_t)sp * DP + ks * 16 + half * 8);
+            }
+            const float wc = valid ? W[sp] : INFINITY;
+            // per row block: MFMAs, then a 16-bit survivor mask per lane
+            bool anyhit = false;
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                if (rb >= (int)nrb) continue;
+                f32x16 acc = {};
+#pragma unroll
+                for (int ks = 0; ks < KS; ks++) {
+                    acc = mfma32<LP>(A[rb][ks], B[ks], acc);
+                    if (SPLIT) {
+                        acc = mfma32<LP>(A[rb][ks], Bl[ks], acc);
+                        acc = mfma32<LP>(Al[rb][ks], B[ks], acc);
+                    }
+                }
+                uint32_t sm = 0;
+#pragma unroll
+                for (int i = 0; i < 16; i++) sm |= (acc[i] >= wq[rb][i] + wc) ? (1u << i) : 0u;
+                if (ballot(sm != 0) == 0) continue;
+                // exact rechecks, lane-divergent; hits set column bits of their row
+#pragma unroll 1
+                while (sm) {
+                    const int i = __builtin_ctz(sm);
+                    sm &= sm - 1;
+                    const uint32_t r = rb * 32 + (i / 4) * 8 + half * 4 + (i % 4);
+                    nrech++;
+                    const T e = dist_mem<T>(xs, ld, q0 + r, sp, a.d, mode, a.ilp);
+                    if (e <= thr) {
+                        nhit++;
+                        anyhit = true;
+                        atomicOr(&s_hm[w][r], 1u << row_l);
+                    }
+                }
+            }
+            if (ballot(anyhit) == 0) continue;
+            if (half == 0) s_cid[w][row_l] = ids[sp];
+            __builtin_amdgcn_wave_barrier();
+            // emit: lane l owns rows l and l + 64; bits in column (= candidate) order
+            uint32_t hm0 = 0, hm1 = 0;
+            if (lane < TQ) { hm0 = s_hm[w][lane]; s_hm[w][lane] = 0; }
+            if (lane + 64 < TQ) { hm1 = s_hm[w][lane + 64]; s_hm[w][lane + 64] = 0; }
+            if (PASS == PASS_COUNT) {
+                if (hm0) s_cnt[w][lane] += __popc(hm0);
+                if (hm1) s_cnt[w][lane + 64] += __popc(hm1);
+            } else if (PASS == PASS_FILL) {
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    uint32_t hm = t ? hm1 : hm0;
+                    if (!hm) continue;
+                    const uint32_t r = lane + 64 * t;
+                    uint64_t pos = s_cur[w][r];
+                    while (hm) {
+                        const int c = __builtin_ctz(hm);
+                        hm &= hm - 1;
+                        if (pos < a.cap) a.out[pos] = s_cid[w][c]; else overflow = true;
+                        pos++;
+                    }
+                    s_cur[w][r] = pos;
+                }
+            } else {
+                const uint32_t mine = __popc(hm0) + __popc(hm1);
+                if (hm0) s_cnt[w][lane] += __popc(hm0);
+                if (hm1) s_cnt[w][lane + 64] += __popc(hm1);
+                uint32_t incl = mine;
+#pragma unroll
+                for (int off = 1; off < GDSJ_WAVE; off <<= 1) {
+                    const uint32_t o = __shfl_up(incl, off, GDSJ_WAVE);
+                    if (lane >= (uint32_t)off) incl += o;
+                }
+                uint32_t p = scur + incl - mine;
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    uint32_t hm = t ? hm1 : hm0;
+                    const uint32_t r = lane + 64 * t;
+                    while (hm) {
+                        const int c = __builtin_ctz(hm);
+                        hm &= hm - 1;
+                        if (p < scap) {
+                            a.slotIds[sbase + p] = s_cid[w][c];
+                            a.slotQ[sbase + p] = (uint8_t)r;
+                        }
+                        p++;
+                    }
+                }
+                scur += uniform(shfl_u32(incl, GDSJ_WAVE - 1));
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (PASS != PASS_FILL) {
+        for (uint32_t r = lane; r < nq; r += GDSJ_WAVE) a.counts[ids[q0 + r]] = s_cnt[w][r];
+        if (a.tileWork && lane == 0) a.tileWork[tile] = work * nq;
+    }
+    if (PASS == PASS_SLOT && lane == 0) {
+        a.slotUsed[tile] = scur;
+        if (scur > scap) a.ovfList[atomicAdd(a.ovfCount, 1u)] = tile;
+    }
+    if (PASS == PASS_FILL && overflow) atomicOr(a.status, 1u);
+    if (stats) {
+        unsigned long long r1 = nrech, r2 = nhit;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            r1 += __shfl_xor(r1, off, GDSJ_WAVE);
+            r2 += __shfl_xor(r2, off, GDSJ_WAVE);
+        }
+        if (lane == 0 && r1) {
+            atomicAdd(&stats[0], r1);
+            atomicAdd(&stats[1], r2);
+        }
+    }
+}
+
+template <typename T, int LP, int KS>
+inline int launch_gram(const GramData &gd, const JoinArgs &a, int mode, int pass, unsigned long long *stats,
+                       hipStream_t s) {
+    const uint32_t blocks = (a.ntiles + 3) / 4;
+    if (blocks == 0) return GDSJ_OK;
+    const T *xs = (const T *)a.xs;
+    const auto *H = (const typename Lp<LP>::scalar *)gd.h.p;
+    const auto *Lo = (const typename Lp<LP>::scalar *)(gd.split ? gd.l.p : gd.h.p);
+    const float *W = gd.w.as<float>();
+    constexpr int SP = LP == 0 ? 1 : 0;
+    if (pass == PASS_FILL)
+        k_gram<T, LP, KS, SP, PASS_FILL><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats);
+    else if (pass == PASS_SLOT)
+        k_gram<T, LP, KS, SP, PASS_SLOT><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats);
+    else
+        k_gram<T, LP, KS, SP, PASS_COUNT><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats);
+    GDSJ_HIP(hipGetLastError());
+    return GDSJ_OK;
+}
+
+template <typename T, int LP>
+inline int gram_dispatch(const GramData &gd, const JoinArgs &a, int mode, int pass, unsigned long long *stats,
+                         hipStream_t s) {
+    switch (gd.dpad / 16) {
+        case 1: return launch_gram<T, LP, 1>(gd, a, mode, pass, stats, s);
+        case 2: return launch_gram<T, LP, 2>(gd, a, mode, pass, stats, s);
+        case 3: return launch_gram<T, LP, 3>(gd, a, mode, pass, stats, s);
+        case 4: return launch_gram<T, LP, 4>(gd, a, mode, pass, stats, s);
+        case 6: return launch_gram<T, LP, 6>(gd, a, mode, pass, stats, s);
+        case 8: return launch_gram<T, LP, 8>(gd, a, mode, pass, stats, s);
+        default: break;
+    }
+    set_error("unsupported padded dims %u for the Gram filter", gd.dpad);
+    return GDSJ_EUNSUPPORTED;
+}
+
+}  // namespace gdsj

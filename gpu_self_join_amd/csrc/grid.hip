@@ -488,7 +488,7 @@ int grid_build(Grid &G, const void *d_data, uint32_t n, uint32_t d, int dtype, d
         set_error("epsilon must be positive and finite");
         return GDSJ_EINVAL;
     }
-    if (tileq == 0 || tileq > 64) tileq = 64;
+    if (tileq == 0 || tileq > 128) tileq = 64;
     G.n = n;
     G.d = d;
     G.k = k;

@@ -98,8 +98,13 @@ int build_runs(Grid &G, hipStream_t s, bool prune, bool *built) {
     return GDSJ_OK;
 }
 
+// tiles hold at most COMPACT_TQ queries (64 for the plain join, up to 128
+// for the Gram filter); the running row cursors live in LDS
+constexpr uint32_t COMPACT_TQ = 128;
+
 __global__ __launch_bounds__(256) void k_compact(const JoinArgs a) {
-    __shared__ uint32_t scnt[256 / GDSJ_WAVE][GDSJ_WAVE];
+    __shared__ unsigned long long scur[256 / GDSJ_WAVE][COMPACT_TQ];
+    __shared__ uint32_t scnt[256 / GDSJ_WAVE][COMPACT_TQ];
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x / GDSJ_WAVE;
     const uint32_t tile = uniform((uint32_t)(blockIdx.x * (blockDim.x / GDSJ_WAVE) + w));
@@ -109,10 +114,12 @@ __global__ __launch_bounds__(256) void k_compact(const JoinArgs a) {
     const uint4 tl = a.tiles[tile];
     const uint32_t q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
     const uint64_t sbase = uniform64(a.slotOff[tile]);
-    uint64_t cur_l = lane < nq ? a.offsets[a.ids[q0 + lane]] : 0ull;
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    scnt[w][lane] = 0;
+    for (uint32_t q = lane; q < COMPACT_TQ; q += GDSJ_WAVE) {
+        scur[w][q] = q < nq ? a.offsets[a.ids[q0 + q]] : 0ull;
+        scnt[w][q] = 0;
+    }
     __builtin_amdgcn_wave_barrier();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     bool overflow = false;
     for (uint32_t e = 0; e < used; e += GDSJ_WAVE) {
         const bool valid = e + lane < used;
@@ -120,22 +127,22 @@ __global__ __launch_bounds__(256) void k_compact(const JoinArgs a) {
         const uint32_t id = valid ? a.slotIds[sbase + e + lane] : 0u;
         uint64_t peers = ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 6; b++) {
+        for (int b = 0; b < 7; b++) {
             const bool bit = (qi >> b) & 1u;
             const uint64_t mb = ballot(bit);
             peers &= bit ? mb : ~mb;
         }
         const uint64_t below = peers & lt;
-        const uint32_t lo = __shfl((uint32_t)cur_l, (int)qi, GDSJ_WAVE);
-        const uint32_t hi = __shfl((uint32_t)(cur_l >> 32), (int)qi, GDSJ_WAVE);
-        const uint64_t pos = (((uint64_t)hi << 32) | lo) + popc64(below);
         if (valid) {
+            const uint64_t pos = scur[w][qi] + popc64(below);
             if (pos < a.cap) a.out[pos] = id; else overflow = true;
             if (below == 0) scnt[w][qi] = popc64(peers);
         }
         __builtin_amdgcn_wave_barrier();
-        cur_l += scnt[w][lane];
-        scnt[w][lane] = 0;
+        for (uint32_t q = lane; q < COMPACT_TQ; q += GDSJ_WAVE) {
+            scur[w][q] += scnt[w][q];
+            scnt[w][q] = 0;
+        }
         __builtin_amdgcn_wave_barrier();
     }
     if (overflow) atomicOr(a.status, 1u);
