@@ -7,4 +7,4 @@ The compute path is libgdsjoin.so (gfx950 HIP kernels behind a C ABI).
 """
 from ._lib import GdsjError, default_params, load  # noqa: F401
 from .gdsjoingpu import gdsjoin, gdsjoinmain  # noqa: F401
-from .selfjoin import JoinResult, SelfJoin  # noqa: F401
+from .selfjoin import BatchJoin, BatchResult, JoinResult, SelfJoin  # noqa: F401

@@ -32,7 +32,10 @@ EXPORTS = [
     "copyResultIntoPythonArray", "gdsj_set_legacy_params", "gdsj_legacy_status",
     "gdsj_mlp_param_count", "gdsj_mlp_feature_count", "gdsj_mlp_forward", "gdsj_mlp_grad",
     "gdsj_model_get", "gdsj_model_set",
+    "gdsj_batch_create", "gdsj_batch_destroy", "gdsj_batch_autotune", "gdsj_batch_run", "gdsj_batch_get_stats",
 ]
+
+ILP_AUTO = 0xFFFFFFFF
 
 
 class GdsjParams(C.Structure):
@@ -57,6 +60,32 @@ class GdsjStats(C.Structure):
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "perm"}
         d["perm"] = list(self.perm)
+        return d
+
+
+class GdsjInstance(C.Structure):
+    _fields_ = [("d_data", C.c_void_p), ("n", C.c_uint32), ("dims", C.c_uint32), ("epsilon", C.c_double)]
+
+
+class GdsjBatchTuning(C.Structure):
+    _fields_ = [("tile_queries", C.c_uint32), ("waves_per_block", C.c_uint32), ("ilp", C.c_uint32),
+                ("lpt", C.c_uint32), ("blocks", C.c_uint32), ("reg_budget", C.c_uint32), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class GdsjBatchStats(C.Structure):
+    _fields_ = [("total_pairs", C.c_uint64), ("candidates", C.c_uint64), ("n_points", C.c_uint64),
+                ("n_tiles", C.c_uint64), ("est_predicted", C.c_uint64), ("est_capacity", C.c_uint64),
+                ("est_overflow_tiles", C.c_uint64), ("slotted", C.c_uint32), ("graph_replays", C.c_uint32),
+                ("graph_captures", C.c_uint32), ("reserved", C.c_uint32),
+                ("ms_build", C.c_double), ("ms_tune", C.c_double), ("ms_train", C.c_double),
+                ("ms_last_run", C.c_double), ("tuning", GdsjBatchTuning)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "tuning"}
+        d["tuning"] = self.tuning.as_dict()
         return d
 
 
@@ -112,6 +141,11 @@ def load():
     L.gdsj_mlp_grad.argtypes = [P, P, P, u32, P, P]
     L.gdsj_model_get.argtypes = [P, P, u32, C.POINTER(C.c_double)]
     L.gdsj_model_set.argtypes = [P, P, u32, C.c_double]
+    L.gdsj_batch_create.argtypes = [i32, C.POINTER(GdsjInstance), u32, i32, C.POINTER(GdsjParams), C.POINTER(P)]
+    L.gdsj_batch_destroy.argtypes = [P]
+    L.gdsj_batch_autotune.argtypes = [P, C.POINTER(GdsjBatchTuning)]
+    L.gdsj_batch_run.argtypes = [P, P, P, P, u64, C.POINTER(u64), P]
+    L.gdsj_batch_get_stats.argtypes = [P, C.POINTER(GdsjBatchStats)]
     _lib = L
     return L
 

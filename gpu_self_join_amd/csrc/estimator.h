@@ -52,6 +52,9 @@ int mlp_init(float *params, uint64_t seed, hipStream_t s);
 
 // Trains the estimator on (features, targets) of the current grid's tiles.
 int estimator_train(Estimator &E, const Grid &G, const uint32_t *counts, hipStream_t s, double *ms);
+// Trains on nt rows of features feat[nt][MLP_F] and targets target[nt] (device).
+int estimator_train_rows(Estimator &E, const float *feat, const float *target, uint32_t nt, hipStream_t s,
+                         double *ms);
 
 // Per-tile slot capacities from the learned model; *predicted = sum of
 // predicted hits.  capOut[t] >= number of queries of t, <= queries x candidates.

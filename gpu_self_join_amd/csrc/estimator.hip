@@ -301,6 +301,16 @@ static uint32_t coprime_step(uint32_t m) {
 int estimator_train(Estimator &E, const Grid &G, const uint32_t *counts, hipStream_t s, double *ms) {
     const uint32_t nt = G.ntiles;
     if (nt == 0 || !G.haveRuns) return GDSJ_OK;
+    GDSJ_TRY(E.feat.ensure((size_t)nt * MLP_F * 4));
+    GDSJ_TRY(E.target.ensure((size_t)nt * 4));
+    GDSJ_TRY(tile_features(G, E.feat.as<float>(), s));
+    GDSJ_TRY(tile_targets(G, counts, E.target.as<float>(), s));
+    return estimator_train_rows(E, E.feat.as<float>(), E.target.as<float>(), nt, s, ms);
+}
+
+int estimator_train_rows(Estimator &E, const float *feat, const float *target, uint32_t nt, hipStream_t s,
+                         double *ms) {
+    if (nt == 0) return GDSJ_OK;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -308,15 +318,11 @@ int estimator_train(Estimator &E, const Grid &G, const uint32_t *counts, hipStre
     GDSJ_TRY(E.params.ensure(MLP_P * 4));
     GDSJ_TRY(E.adam_m.ensure(MLP_P * 4));
     GDSJ_TRY(E.adam_v.ensure(MLP_P * 4));
-    GDSJ_TRY(E.feat.ensure((size_t)nt * MLP_F * 4));
-    GDSJ_TRY(E.target.ensure((size_t)nt * 4));
-    GDSJ_TRY(tile_features(G, E.feat.as<float>(), s));
-    GDSJ_TRY(tile_targets(G, counts, E.target.as<float>(), s));
     const uint32_t m = std::min<uint32_t>(nt, 65536);
     GDSJ_TRY(E.sample.ensure((size_t)m * (MLP_F + 1) * 4 + 256));
     float *sx = E.sample.as<float>();
     float *st = sx + (size_t)m * MLP_F;
-    k_gather_rows<<<ceil_div(m, 256), 256, 0, s>>>(E.feat.as<float>(), E.target.as<float>(), nt, m, sx, st);
+    k_gather_rows<<<ceil_div(m, 256), 256, 0, s>>>(feat, target, nt, m, sx, st);
     if (!E.init) {
         GDSJ_TRY(mlp_init(E.params.as<float>(), 0x5eed1234ull, s));
         // output bias starts at the mean target

@@ -260,7 +260,7 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     }
     if (PASS == PASS_SLOT && lane == 0) {
         a.slotUsed[tile] = scur;
-        if (scur > scap) a.ovfList[atomicAdd(a.ovfCount, 1u)] = tile;
+        if (scur > scap) a.ovfList[atomicAdd(a.ovfCount, 1u)] = a.tileBase + tile;
     }
     if (PASS == PASS_FILL && overflow) atomicOr(a.status, 1u);
     if (stats) {

@@ -51,8 +51,9 @@ struct JoinArgs {
     uint32_t *slotUsed;       // hits per tile (may exceed slotCap: overflow)
     uint32_t *slotIds;
     uint8_t *slotQ;           // query index inside the tile of each slot entry
-    uint32_t *ovfList;        // tiles whose slot overflowed
+    uint32_t *ovfList;        // tiles whose slot overflowed (tileBase + tile)
     uint32_t *ovfCount;
+    uint32_t tileBase;        // first global tile id of this join in a batch (0 alone)
     uint32_t *status;  // [0] set when a fill write would exceed cap
 };
 

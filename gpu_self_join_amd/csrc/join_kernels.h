@@ -129,16 +129,15 @@ __device__ __forceinline__ T dist_mem(const T *__restrict__ xs, uint64_t ld, uin
     return acc;
 }
 
-template <typename T, int D, int MODE, int ILPV, int PASS>
-__global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
-                                              const JoinArgs a) {
-    constexpr int CMAX = CMaxFor<T, D>::value;
+// One work tile of the join (one wave): the body shared by the per-join
+// kernel below and the persistent batched kernel (batch.hip).
+// CM: candidate chunks held in VGPRs (0: CMaxFor's default).
+template <typename T, int D, int MODE, int ILPV, int PASS, int CM = 0>
+__device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
+                                          const JoinArgs &a, const uint32_t tile) {
+    constexpr int CMAX = CM > 0 ? CM : CMaxFor<T, D>::value;
     constexpr bool NEED_IDS = PASS != PASS_COUNT;
     const uint32_t lane = lane_id();
-    const uint32_t lblock = a.xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const uint32_t tslot = uniform((uint32_t)(lblock * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
-    if (tslot >= a.ntiles) return;
-    const uint32_t tile = a.tileList ? uniform(a.tileList[tslot]) : tslot;
     const uint64_t ld = a.ld;
     const T thr = (T)a.thr;
     const uint4 tl = a.tiles[tile];
@@ -282,9 +281,19 @@ __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const ui
     if (PASS != PASS_FILL && a.tileWork && lane == 0) a.tileWork[tile] = work * nq;
     if (PASS == PASS_SLOT && lane == 0) {
         a.slotUsed[tile] = scur;
-        if (scur > scap) a.ovfList[atomicAdd(a.ovfCount, 1u)] = tile;
+        if (scur > scap) a.ovfList[atomicAdd(a.ovfCount, 1u)] = a.tileBase + tile;
     }
     if (PASS == PASS_FILL && overflow) atomicOr(a.status, 1u);
+}
+
+template <typename T, int D, int MODE, int ILPV, int PASS>
+__global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
+                                              const JoinArgs a) {
+    const uint32_t lblock = a.xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t tslot = uniform((uint32_t)(lblock * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
+    if (tslot >= a.ntiles) return;
+    const uint32_t tile = a.tileList ? uniform(a.tileList[tslot]) : tslot;
+    join_tile<T, D, MODE, ILPV, PASS>(xs, ids, a, tile);
 }
 
 }  // namespace gdsj

@@ -8,6 +8,7 @@
 // compact_slots: one wave per tile moves the tile's slot entries (neighbor
 // id + query index, written in candidate order) into the CSR rows with a
 // stable partition by query index.
+#include "compact.h"
 #include "join.h"
 #include "neighbors.h"
 
@@ -98,54 +99,10 @@ int build_runs(Grid &G, hipStream_t s, bool prune, bool *built) {
     return GDSJ_OK;
 }
 
-// tiles hold at most COMPACT_TQ queries (64 for the plain join, up to 128
-// for the Gram filter); the running row cursors live in LDS
-constexpr uint32_t COMPACT_TQ = 128;
-
 __global__ __launch_bounds__(256) void k_compact(const JoinArgs a) {
-    __shared__ unsigned long long scur[256 / GDSJ_WAVE][COMPACT_TQ];
-    __shared__ uint32_t scnt[256 / GDSJ_WAVE][COMPACT_TQ];
-    const uint32_t lane = lane_id();
-    const uint32_t w = threadIdx.x / GDSJ_WAVE;
-    const uint32_t tile = uniform((uint32_t)(blockIdx.x * (blockDim.x / GDSJ_WAVE) + w));
+    const uint32_t tile = uniform((uint32_t)(blockIdx.x * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
     if (tile >= a.ntiles) return;
-    const uint32_t used = uniform(a.slotUsed[tile]);
-    if (used > uniform(a.slotCap[tile])) return;  // overflowed: completed by the fill pass
-    const uint4 tl = a.tiles[tile];
-    const uint32_t q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
-    const uint64_t sbase = uniform64(a.slotOff[tile]);
-    for (uint32_t q = lane; q < COMPACT_TQ; q += GDSJ_WAVE) {
-        scur[w][q] = q < nq ? a.offsets[a.ids[q0 + q]] : 0ull;
-        scnt[w][q] = 0;
-    }
-    __builtin_amdgcn_wave_barrier();
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    bool overflow = false;
-    for (uint32_t e = 0; e < used; e += GDSJ_WAVE) {
-        const bool valid = e + lane < used;
-        const uint32_t qi = valid ? (uint32_t)a.slotQ[sbase + e + lane] : 0u;
-        const uint32_t id = valid ? a.slotIds[sbase + e + lane] : 0u;
-        uint64_t peers = ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 7; b++) {
-            const bool bit = (qi >> b) & 1u;
-            const uint64_t mb = ballot(bit);
-            peers &= bit ? mb : ~mb;
-        }
-        const uint64_t below = peers & lt;
-        if (valid) {
-            const uint64_t pos = scur[w][qi] + popc64(below);
-            if (pos < a.cap) a.out[pos] = id; else overflow = true;
-            if (below == 0) scnt[w][qi] = popc64(peers);
-        }
-        __builtin_amdgcn_wave_barrier();
-        for (uint32_t q = lane; q < COMPACT_TQ; q += GDSJ_WAVE) {
-            scur[w][q] += scnt[w][q];
-            scnt[w][q] = 0;
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
-    if (overflow) atomicOr(a.status, 1u);
+    compact_tile(a, tile);
 }
 
 int compact_slots(const JoinArgs &a, hipStream_t s) {

@@ -160,6 +160,79 @@ int gdsj_model_set(gdsj_handle *h, const float *params, uint32_t count, double a
 /* Thread-local text of the last error. */
 const char *gdsj_last_error(void);
 
+/* ---- batched joins ------------------------------------------------------ */
+/* Many independent self-joins of uneven n, dims and epsilon run as one job:
+ * each pass is one persistent launch over a device work queue of
+ * (instance, tile) items, and a run is one hipGraph, captured on the first
+ * run for a given output and replayed afterwards.  No counterpart in the
+ * reference (one join per GDSJoinPy call, main.cu:360; the CLI joins one
+ * file, main.cu:76-254): each instance's rows are exactly what
+ * gdsj_join_device returns for that instance alone with the same params. */
+#define GDSJ_ILP_AUTO 0xFFFFFFFFu  /* params.ilp for batches: the autotuner picks ILP 0 or 8;
+                                      the choice is part of the result contract and is
+                                      reported in gdsj_batch_tuning.ilp */
+
+typedef struct gdsj_instance {
+    const void *d_data;  /* n x dims row-major DTYPE, device memory (kept valid and unchanged) */
+    uint32_t n;
+    uint32_t dims;
+    double epsilon;
+} gdsj_instance;
+
+typedef struct gdsj_batch_tuning {
+    uint32_t tile_queries;    /* queries per work tile (16, 32, 64) */
+    uint32_t waves_per_block; /* 1, 2 or 4 wave64s per workgroup */
+    uint32_t ilp;             /* ILP in force */
+    uint32_t lpt;             /* 1: queue hands out the heaviest tiles first */
+    uint32_t blocks;          /* persistent grid size of the last counting pass */
+    uint32_t reg_budget;      /* 24 or 48 coordinate words per lane of candidates in registers */
+    double ms;                /* counting-pass time the autotuner measured for this choice */
+} gdsj_batch_tuning;
+
+typedef struct gdsj_batch_stats {
+    uint64_t total_pairs;        /* all instances, self-pairs included */
+    uint64_t candidates;         /* point comparisons */
+    uint64_t n_points;           /* sum of n */
+    uint64_t n_tiles;
+    uint64_t est_predicted;      /* learned: predicted pairs */
+    uint64_t est_capacity;       /* learned: result slots (pairs) */
+    uint64_t est_overflow_tiles; /* learned: tiles completed by the exact fill */
+    uint32_t slotted;            /* 1: the last run used learned slots */
+    uint32_t graph_replays;      /* runs so far */
+    uint32_t graph_captures;     /* graphs captured so far (one per output buffers and mode) */
+    uint32_t reserved;
+    double ms_build;             /* grids of every instance (set-up) */
+    double ms_tune;              /* autotuner */
+    double ms_train;             /* estimator training */
+    double ms_last_run;          /* the last run's graph (device time) */
+    gdsj_batch_tuning tuning;
+} gdsj_batch_stats;
+
+typedef struct gdsj_batch gdsj_batch;
+
+/* Builds every instance's grid, runs and tiles on `device` (index
+ * construction).  p is shared: k = 0 follows each instance's dims; ilp may
+ * be GDSJ_ILP_AUTO; filter PLAIN or AUTO (batches use the plain kernels);
+ * estimator EXACT, or LEARNED (the first run is exact and trains the MLP,
+ * later runs are single slotted passes with the exact fill for overflowed
+ * tiles). */
+int gdsj_batch_create(int device, const gdsj_instance *inst, uint32_t ninst, int dtype,
+                      const gdsj_params *p, gdsj_batch **out);
+int gdsj_batch_destroy(gdsj_batch *b);
+/* Times the counting pass for every tile size, block size, register budget
+ * and queue order (and ILP 0/8 under GDSJ_ILP_AUTO) and keeps the fastest;
+ * rebuilds the tiles when the tile size changes.  chosen may be NULL. */
+int gdsj_batch_autotune(gdsj_batch *b, gdsj_batch_tuning *chosen);
+/* Runs every join.  Row r of instance i is row rowBase_i + r with
+ * rowBase_i = n_0 + ... + n_{i-1}: d_counts uint32[sum n], d_offsets
+ * uint64[sum n + 1] (positions in d_ids), d_ids uint32[ids_cap] holding
+ * instance-local point ids.  *total = all pairs; GDSJ_EOVERFLOW when
+ * total > ids_cap (counts and offsets valid).  stream: hipStream_t or NULL;
+ * returns after the stream drained. */
+int gdsj_batch_run(gdsj_batch *b, uint32_t *d_counts, uint64_t *d_offsets, uint32_t *d_ids,
+                   uint64_t ids_cap, uint64_t *total, void *stream);
+int gdsj_batch_get_stats(gdsj_batch *b, gdsj_batch_stats *out);
+
 /* ---- legacy drop-in symbols -------------------------------------------- */
 /* main.cu:360.  DTYPE float in this symbol; GDSJoinPyDouble for double.
  * Dims and k come from the arguments and gdsj_set_legacy_params() instead
