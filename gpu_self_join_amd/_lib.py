@@ -33,6 +33,7 @@ EXPORTS = [
     "gdsj_mlp_param_count", "gdsj_mlp_feature_count", "gdsj_mlp_forward", "gdsj_mlp_grad",
     "gdsj_model_get", "gdsj_model_set",
     "gdsj_batch_create", "gdsj_batch_destroy", "gdsj_batch_autotune", "gdsj_batch_run", "gdsj_batch_get_stats",
+    "gdsj_csv_load",
 ]
 
 ILP_AUTO = 0xFFFFFFFF
@@ -146,6 +147,7 @@ def load():
     L.gdsj_batch_autotune.argtypes = [P, C.POINTER(GdsjBatchTuning)]
     L.gdsj_batch_run.argtypes = [P, P, P, P, u64, C.POINTER(u64), P]
     L.gdsj_batch_get_stats.argtypes = [P, C.POINTER(GdsjBatchStats)]
+    L.gdsj_csv_load.argtypes = [C.c_char_p, i32, P, u64, C.POINTER(u64)]
     _lib = L
     return L
 
@@ -159,6 +161,19 @@ def check(rc):
     if rc != GDSJ_OK:
         raise GdsjError(rc, last_error())
     return rc
+
+
+def csv_load(path, dtype="float"):
+    """Values of a text dataset read as the reference's importNDDataset reads them (flat array)."""
+    import numpy as np
+    dt = DTYPE_F64 if dtype in ("double", np.float64) else DTYPE_F32
+    npdt = np.float64 if dt == DTYPE_F64 else np.float32
+    L = load()
+    n = C.c_uint64(0)
+    check(L.gdsj_csv_load(os.fsencode(path), dt, None, 0, C.byref(n)))
+    out = np.empty(n.value, dtype=npdt)
+    check(L.gdsj_csv_load(os.fsencode(path), dt, out.ctypes.data if n.value else None, n.value, C.byref(n)))
+    return out
 
 
 def default_params(**kw):

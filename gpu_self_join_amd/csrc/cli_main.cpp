@@ -1,0 +1,177 @@
+// cli_main.cpp -- the command-line join: gdsjoin_main <file> <eps> <dim> 3
+//
+// Drop-in for the reference's `main <dataset> <epsilon> <dims> <mode>`
+// (main.cu:66-120, mode 3 at main.cu:178-331): imports the text dataset
+// (gdsj_csv_load, import_dataset.cpp:49-95), runs the grid self-join on the
+// GPU through the C ABI, prints the time and the total, and appends one line
+// to gpu_stats.txt in the reference's format (main.cu:257-262).  The
+// reference fixes dims, NUMINDEXEDDIM, DTYPE and the optimisation switches
+// at compile time (params.h); here they are options:
+//   --dtype float|double (default float)  --k K (default min(dim, 6))
+//   --ilp I  --shortcircuit 0|1  --reorder 0|1  --stamp 0|1  --queryreorder 0|1
+//   --estimator exact|learned|sample  --filter plain|gram|auto  --device D
+//   --print   print every point's sorted neighbor list (PRINTNEIGHBORTABLE)
+// Errors exit with a non-zero status (the reference prints and returns 0).
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "gdsjoin.h"
+
+static int usage() {
+    fprintf(stderr,
+            "usage: gdsjoin_main <dataset file> <epsilon> <dims> <mode> [options]\n"
+            "  mode 3: GPU grid self-join (the only mode of this build)\n"
+            "  options: --dtype float|double --k K --ilp I --shortcircuit 0|1 --reorder 0|1\n"
+            "           --stamp 0|1 --queryreorder 0|1 --estimator exact|learned|sample\n"
+            "           --filter plain|gram|auto --device D --print\n");
+    return 2;
+}
+
+static int fail(const char *what, int rc) {
+    fprintf(stderr, "\nERROR: %s: %s (status %d)\n", what, gdsj_last_error(), rc);
+    return 1;
+}
+
+int main(int argc, char **argv) {
+    if (argc < 5) {
+        fprintf(stderr, "\n\nIncorrect number of input parameters.  \n"
+                        "Should be dataset file, epsilon, number of dimensions, searchmode\n");
+        return usage();
+    }
+    const char *fname = argv[1];
+    const double eps_arg = atof(argv[2]);  // main.cu:95: DTYPE epsilon = atof(...)
+    const int ndim = atoi(argv[3]);
+    const int mode = atoi(argv[4]);
+    int dtype = GDSJ_DTYPE_F32, device = 0, print = 0;
+    gdsj_params p;
+    gdsj_params_default(&p);
+    for (int i = 5; i < argc; i++) {
+        const char *a = argv[i];
+        const char *v = i + 1 < argc ? argv[i + 1] : nullptr;
+        if (!strcmp(a, "--print")) {
+            print = 1;
+            continue;
+        }
+        if (!v) return usage();
+        i++;
+        if (!strcmp(a, "--dtype")) {
+            if (!strcmp(v, "float")) dtype = GDSJ_DTYPE_F32;
+            else if (!strcmp(v, "double")) dtype = GDSJ_DTYPE_F64;
+            else return usage();
+        } else if (!strcmp(a, "--k")) p.k = (uint32_t)atoi(v);
+        else if (!strcmp(a, "--ilp")) p.ilp = (uint32_t)atoi(v);
+        else if (!strcmp(a, "--shortcircuit")) p.shortcircuit = (uint32_t)atoi(v);
+        else if (!strcmp(a, "--reorder")) p.reorder = (uint32_t)atoi(v);
+        else if (!strcmp(a, "--stamp")) p.stamp = (uint32_t)atoi(v);
+        else if (!strcmp(a, "--queryreorder")) p.queryreorder = (uint32_t)atoi(v);
+        else if (!strcmp(a, "--device")) device = atoi(v);
+        else if (!strcmp(a, "--estimator")) {
+            if (!strcmp(v, "exact")) p.estimator = GDSJ_EST_EXACT;
+            else if (!strcmp(v, "learned")) p.estimator = GDSJ_EST_LEARNED;
+            else if (!strcmp(v, "sample")) p.estimator = GDSJ_EST_SAMPLE;
+            else return usage();
+        } else if (!strcmp(a, "--filter")) {
+            if (!strcmp(v, "plain")) p.filter = GDSJ_FILTER_PLAIN;
+            else if (!strcmp(v, "gram")) p.filter = GDSJ_FILTER_GRAM;
+            else if (!strcmp(v, "auto")) p.filter = GDSJ_FILTER_AUTO;
+            else return usage();
+        } else {
+            return usage();
+        }
+    }
+    if (ndim <= 0) {
+        fprintf(stderr, "\nERROR: the number of dimensions must be positive\n");
+        return 1;
+    }
+    if (mode != 3) {
+        fprintf(stderr, "\nERROR: search mode %d is not part of this build (mode 3: GPU grid join)\n", mode);
+        return 2;
+    }
+    // DTYPE epsilon = atof(...): the double value rounded once to DTYPE
+    const double eps = dtype == GDSJ_DTYPE_F32 ? (double)(float)eps_arg : eps_arg;
+    printf("\nDataset file: %s", fname);
+    printf("\nEpsilon: %f", eps);
+    printf("\nNumber of dimensions (NDIM): %d\n", ndim);
+
+    uint64_t nvals = 0;
+    int rc = gdsj_csv_load(fname, dtype, nullptr, 0, &nvals);
+    if (rc) return fail("import", rc);
+    const size_t es = dtype == GDSJ_DTYPE_F64 ? 8 : 4;
+    std::vector<unsigned char> data((size_t)nvals * es + 8);
+    rc = gdsj_csv_load(fname, dtype, data.data(), nvals, &nvals);
+    if (rc) return fail("import", rc);
+    const uint64_t npts = nvals / (uint64_t)ndim;
+    if (npts > 0xFFFFFFFFull) {
+        fprintf(stderr, "\nERROR: more than 2^32 - 1 points\n");
+        return 1;
+    }
+    printf("\nData import: Total size of all data (1-D) vect (number of points * GPUNUMDIM): %llu",
+           (unsigned long long)nvals);
+    printf("\nData import: Total data points: %llu", (unsigned long long)npts);
+
+    gdsj_handle *h = nullptr;
+    rc = gdsj_create(device, &h);
+    if (rc) return fail("gdsj_create", rc);
+    // warm-up (main.cu:193): a tiny join initialises the device and the kernels
+    {
+        float w[8] = {0, 0, 1, 1, 2, 2, 3, 3};
+        uint32_t wc[4];
+        uint64_t wt = 0;
+        gdsj_params wp;
+        gdsj_params_default(&wp);
+        rc = gdsj_join_host(h, w, 4, 2, GDSJ_DTYPE_F32, 1.5, &wp, wc, &wt);
+        if (rc) return fail("warm-up", rc);
+    }
+    std::vector<uint32_t> counts(npts ? npts : 1);
+    uint64_t total = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = gdsj_join_host(h, data.data(), (uint32_t)npts, (uint32_t)ndim, dtype, eps, &p, counts.data(), &total);
+    const auto t1 = std::chrono::steady_clock::now();
+    if (rc) {
+        gdsj_destroy(h);
+        return fail("join", rc);
+    }
+    const double secs = std::chrono::duration<double>(t1 - t0).count();
+    gdsj_stats st;
+    gdsj_get_stats(h, &st);
+    printf("\nTotal neighbors: %llu", (unsigned long long)total);
+    printf("\nTime: %f", secs);
+    printf("\nDevice time (ms): grid %.3f, count %.3f, scan %.3f, fill %.3f, total %.3f\n", st.ms_grid,
+           st.ms_count, st.ms_scan, st.ms_fill, st.ms_total);
+
+    const uint32_t k = p.k ? p.k : (uint32_t)std::min(ndim, 6);
+    std::ofstream stats("gpu_stats.txt", std::ios::app);
+    stats << secs << ", " << fname << ", " << eps << ", " << total
+          << ", GPUNUMDIM/NUMINDEXEDDIM/ILP/STAMP/SORT/REORDER/SHORTCIRCUIT/QUERYREORDER/DTYPE(float/double): "
+          << ndim << ", " << k << ", " << p.ilp << ", " << p.stamp << ", " << 0 << ", " << p.reorder << ", "
+          << p.shortcircuit << ", " << p.queryreorder << ", " << (dtype == GDSJ_DTYPE_F64 ? "double" : "float")
+          << std::endl;
+    stats.close();
+
+    if (print) {
+        std::vector<uint32_t> ids(total ? total : 1);
+        rc = gdsj_result_copy(h, ids.data(), total);
+        if (rc) {
+            gdsj_destroy(h);
+            return fail("result copy", rc);
+        }
+        uint64_t off = 0;
+        for (uint64_t i = 0; i < npts; i++) {
+            std::sort(ids.begin() + (long)off, ids.begin() + (long)(off + counts[i]));
+            printf("\npoint id: %llu, neighbors: ", (unsigned long long)i);
+            for (uint32_t j = 0; j < counts[i]; j++) printf("%u,", ids[off + j]);
+            off += counts[i];
+        }
+        printf("\n");
+    }
+    gdsj_destroy(h);
+    return 0;
+}

@@ -160,6 +160,14 @@ int gdsj_model_set(gdsj_handle *h, const float *params, uint32_t count, double a
 /* Thread-local text of the last error. */
 const char *gdsj_last_error(void);
 
+/* ---- dataset import ------------------------------------------------------ */
+/* Replaces importNDDataset (import_dataset.cpp:49-95): the text is cut at
+ * every ',' and each piece read with C++ formatted extraction into DTYPE
+ * (decimal -> DTYPE directly) until an extraction fails.  *count = number of
+ * values read; out (may be NULL with cap 0) receives the first cap of them.
+ * Host only: needs no GPU. */
+int gdsj_csv_load(const char *path, int dtype, void *out, uint64_t cap, uint64_t *count);
+
 /* ---- batched joins ------------------------------------------------------ */
 /* Many independent self-joins of uneven n, dims and epsilon run as one job:
  * each pass is one persistent launch over a device work queue of

@@ -42,6 +42,10 @@ def lib():
             getattr(L, "orc_reorder_perm_" + sfx).argtypes = [P, C.c_uint32, C.c_uint32, P, P]
             getattr(L, "orc_parse_text_" + sfx).argtypes = [C.c_char_p, C.c_size_t, P, C.c_size_t]
             getattr(L, "orc_parse_text_" + sfx).restype = C.c_size_t
+            getattr(L, "cpu_bruteforce_" + sfx).argtypes = [P, C.c_uint32, C.c_uint32, ct, P, C.c_uint32, C.c_int,
+                                                            P, C.POINTER(C.c_double)]
+            getattr(L, "cpu_rtree_" + sfx).argtypes = [P, C.c_uint32, C.c_uint32, ct, P, C.c_uint32, P,
+                                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
         for sfx, ct in (("f32", C.c_float), ("f64", C.c_double)):
             f = getattr(L, "orc_acc_" + sfx)
             f.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
@@ -204,3 +208,37 @@ def csr_hashes(counts, ids):
         h = cs[offs[1:]] - cs[offs[:-1]]
     h[~nz] = 0
     return h
+
+
+# ------------------------------------------------------------ CPU baselines --
+# The reference's CPU self-join paths restated (cpu_baselines.c), for timing
+# beside the GPU: unfused filter arithmetic, so boundary pairs may differ
+# from the grid contract.
+
+def cpu_bruteforce(data, eps, qids, nthreads=1):
+    """Counts of the queries qids against all points; returns (counts, seconds)."""
+    data = _prep(data)
+    n, d = data.shape
+    q = np.ascontiguousarray(qids, dtype=np.uint32)
+    counts = np.zeros(len(q), np.uint32)
+    sec = C.c_double(0)
+    rc = getattr(lib(), "cpu_bruteforce_" + _sfx(data.dtype))(_ptr(data), n, d, _scalar(data.dtype, eps), _ptr(q),
+                                                              len(q), int(nthreads), _ptr(counts), C.byref(sec))
+    if rc != 0:
+        raise RuntimeError("cpu_bruteforce failed: %d" % rc)
+    return counts, sec.value
+
+
+def cpu_rtree(data, eps, qids):
+    """Sequential R-tree: builds over all points, queries qids.
+    Returns (counts, build_seconds, query_seconds)."""
+    data = _prep(data)
+    n, d = data.shape
+    q = np.ascontiguousarray(qids, dtype=np.uint32)
+    counts = np.zeros(len(q), np.uint32)
+    tb, tq = C.c_double(0), C.c_double(0)
+    rc = getattr(lib(), "cpu_rtree_" + _sfx(data.dtype))(_ptr(data), n, d, _scalar(data.dtype, eps), _ptr(q), len(q),
+                                                         _ptr(counts), C.byref(tb), C.byref(tq))
+    if rc != 0:
+        raise RuntimeError("cpu_rtree failed: %d" % rc)
+    return counts, tb.value, tq.value

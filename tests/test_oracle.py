@@ -174,3 +174,21 @@ def test_parse_text_direct_decimal_to_float():
     v = O.parse_text("0.1,0.2\n3.4028235e38, 1e-45\n", np.float32)
     assert v.dtype == np.float32
     assert list(v) == [np.float32("0.1"), np.float32("0.2"), np.float32(3.4028235e38), np.float32(1e-45)]
+
+
+def test_cpu_baselines_rtree_equals_bruteforce():
+    """The restated reference CPU paths (timing baselines) find the same
+    counts: R-tree search + filter == all-pairs scan with the same filter."""
+    rng = np.random.default_rng(21)
+    for d, dt, eps in ((2, np.float32, 0.3), (5, np.float64, 1.1), (12, np.float32, 2.5)):
+        x = (rng.random((3000, d)) * 6).astype(dt)
+        q = np.arange(0, 3000, 3, dtype=np.uint32)
+        e = float(np.asarray(eps, dtype=dt))
+        cb, _ = O.cpu_bruteforce(x, e, q, 2)
+        cr, _, _ = O.cpu_rtree(x, e, q)
+        assert np.array_equal(cb, cr)
+        assert np.all(cb >= 1)
+        # the grid contract fuses its arithmetic: counts may differ only at
+        # the epsilon boundary, which random data almost never hits
+        g = O.join(x, e, k=min(d, 6), with_ids=False)["counts"][q]
+        assert int(np.abs(g.astype(np.int64) - cb).sum()) <= 2
