@@ -113,6 +113,26 @@ def copy_probe(nbytes, reps=5):
                       "read_plus_write_GBps": 2 * n * 4 / (best * 1e-3) / 1e9}))
 
 
+def filter_error(pts, eps, base):
+    """One untimed join with the Gram filter's error measurement: the max over
+    rechecked pairs of |Gram-form squared distance - FP64 squared distance|,
+    relative to eps^2, and the band hit rate (rechecked pairs that are
+    neighbors)."""
+    from gpu_self_join_amd import SelfJoin
+    sj = SelfJoin(0)
+    try:
+        r = sj.join(pts, eps, filter=1, filter_stats=1, estimator=0, **{k: v for k, v in base.items()
+                                                                        if k not in ("filter", "estimator")})
+        st = r.stats
+        return {"max_abs_err_over_eps2": st["filter_max_err"], "rechecked_pairs": st["filter_rechecks"],
+                "band_hits": st["filter_hits"],
+                "band_hit_rate": st["filter_hits"] / max(1, st["filter_rechecks"]),
+                "candidates": st["candidates"],
+                "filtered_out_frac": 1.0 - st["filter_rechecks"] / max(1, st["candidates"])}
+    finally:
+        sj.close()
+
+
 def run_variant(sj, pts, eps, kw, steps, warmup, total=None):
     """Times `steps` joins (after `warmup`) with output into preallocated buffers."""
     import torch
@@ -346,6 +366,7 @@ def main():
         "variants": variants,
         "grid": {"nonempty_cells": s["n_nonempty_cells"], "tiles": s["n_tiles"], "candidates": s["candidates"]},
         "speedup_vs_plain": (results["plain"]["ms_per_step"] / r["ms_per_step"]) if "plain" in results else None,
+        "filter_error": filter_error(pts, eps, base) if d >= 16 else None,
         "roofline": roof,
     }
     if args.no_cpu_baseline:

@@ -217,7 +217,8 @@ static JoinArgs make_args(gdsj_handle *h) {
 static int run_pass(gdsj_handle *h, const JoinArgs &a, int pass) {
     if (h->use_gram)
         return gram_join_pass(h->G, h->gram, a, h->mode, pass,
-                              pass == PASS_FILL ? nullptr : h->gramStats.as<unsigned long long>(), h->stream);
+                              pass == PASS_FILL ? nullptr : h->gramStats.as<unsigned long long>(), h->stream,
+                              h->p.filter_stats != 0);
     return join_pass(h->G, a, h->mode, pass, h->stream);
 }
 
@@ -365,7 +366,7 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     if (h->use_gram) {
         GDSJ_TRY(gram_prepare(h->gram, h->G, dtype == GDSJ_DTYPE_F64 ? 1 : 0, s));
         GDSJ_TRY(h->gramStats.ensure(64));
-        GDSJ_HIP(hipMemsetAsync(h->gramStats.p, 0, 16, s));
+        GDSJ_HIP(hipMemsetAsync(h->gramStats.p, 0, 24, s));
     }
     {
         bool built = false;
@@ -388,7 +389,9 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     // ---- counting: slotted single pass when an estimator sizes the slots,
     //      else the exact counting pass
     bool slotted = false;
-    if (h->p.estimator != GDSJ_EST_EXACT) GDSJ_TRY(slotted_pass(h, a, &slotted));
+    // filter-error measurement runs in the exact counting pass
+    if (h->p.estimator != GDSJ_EST_EXACT && !(h->use_gram && h->p.filter_stats))
+        GDSJ_TRY(slotted_pass(h, a, &slotted));
     if (!slotted) {
         GDSJ_HIP(hipEventRecord(h->ev[1], s));
         GDSJ_HIP(hipEventRecord(h->ev[2], s));
@@ -400,8 +403,8 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     GDSJ_HIP(hipEventRecord(h->ev[4], s));
     uint64_t *candSum = h->status.as<uint64_t>() + 4;
     GDSJ_TRY(sum_u64(a.tileWork, h->G.ntiles, candSum, s));
-    uint64_t hres[3] = {0, 0, 0}, gst[2] = {0, 0};
-    if (h->use_gram) GDSJ_HIP(hipMemcpyAsync(gst, h->gramStats.p, 16, hipMemcpyDeviceToHost, s));
+    uint64_t hres[3] = {0, 0, 0}, gst[3] = {0, 0, 0};
+    if (h->use_gram) GDSJ_HIP(hipMemcpyAsync(gst, h->gramStats.p, 24, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipMemcpyAsync(&hres[0], offs + n, 8, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipMemcpyAsync(&hres[1], candSum, 8, hipMemcpyDeviceToHost, s));
     if (slotted) GDSJ_HIP(hipMemcpyAsync(&hres[2], h->scalars.p, 4, hipMemcpyDeviceToHost, s));
@@ -420,6 +423,7 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     h->st.est_overflow_tiles = h->novf;
     h->st.filter_rechecks = gst[0];
     h->st.filter_hits = gst[1];
+    memcpy(&h->st.filter_max_err, &gst[2], 8);
     for (uint32_t i = 0; i < dims && i < 64; i++) h->st.perm[i] = h->G.perm[i];
     h->st.ms_grid = ms_between(h->ev[0], h->ev[1]);
     h->st.ms_estimate = ms_between(h->ev[1], h->ev[2]);

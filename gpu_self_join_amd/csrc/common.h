@@ -32,7 +32,8 @@ void set_error(const char *fmt, ...);
     } while (0)
 
 // ------------------------------------------------------- wave primitives --
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// the wave's predicate mask (the compare result itself, no 0/1 materialisation)
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));

@@ -22,7 +22,8 @@ struct GramData {
     DevBuf h;        // [n][dpad] bf16 or fp16: leading part of sigma (x - o)
     DevBuf l;        // [n][dpad] the remainder in the same format (split form only)
     DevBuf w;        // [n] float
-    DevBuf scratch;  // per-dim sums / maxima, stats
+    DevBuf n2;       // [n] double: N = |h + l|^2 (filter-error measurement)
+    DevBuf scratch;  // per-dim means [d], max deviations [d], sigma
     uint32_t dpad = 0;
     int lowp = 0;    // 0 bf16, 1 fp16
     int split = 0;   // 1: S = h.h' + h.l' + l.h' (three MFMAs per k-step)
@@ -35,9 +36,11 @@ bool gram_supported(uint32_t d);
 uint32_t gram_tile_queries(uint32_t d, int lowp);
 // Builds h and w for the grid's points; lowp: 0 bf16, 1 fp16.
 int gram_prepare(GramData &gd, const Grid &G, int lowp, hipStream_t s);
-// One pass (JoinPass) of the filtered join (tiles of <= 32 queries).
-// stats[0] += rechecked pairs, stats[1] += rechecked pairs that are hits.
+// One pass (JoinPass) of the filtered join.  stats[0] += rechecked pairs,
+// stats[1] += rechecked pairs that are hits; with measure_err, stats[2] =
+// max over rechecked pairs of |Gram estimate - FP64 squared distance| / eps^2
+// (double bits).
 int gram_join_pass(const Grid &G, const GramData &gd, const JoinArgs &a, int mode, int pass,
-                   unsigned long long *stats, hipStream_t s);
+                   unsigned long long *stats, hipStream_t s, bool measure_err = false);
 
 }  // namespace gdsj

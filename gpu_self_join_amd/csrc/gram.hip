@@ -21,6 +21,7 @@ void GramData::release() {
     h.release();
     l.release();
     w.release();
+    n2.release();
     scratch.release();
     dpad = 0;
 }
@@ -95,7 +96,8 @@ template <typename T, int LP>
 __global__ void k_gram_convert(const T *__restrict__ xs, uint64_t ld, uint32_t n, uint32_t d, uint32_t dpad,
                                const double *__restrict__ mean, const double *__restrict__ maxdev, double eps_p,
                                double cfac, int split, typename Lp<LP>::scalar *__restrict__ H,
-                               typename Lp<LP>::scalar *__restrict__ Lo, float *__restrict__ W) {
+                               typename Lp<LP>::scalar *__restrict__ Lo, float *__restrict__ W,
+                               double *__restrict__ N2, double *__restrict__ sigOut) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     double sigma = 1.0;
@@ -131,6 +133,8 @@ __global__ void k_gram_convert(const T *__restrict__ xs, uint64_t ld, uint32_t n
     const double k = sigma * eps_p * 0.5 + e;
     const double w = 0.5 * (N * (1.0 - 0x1p-40) - cfac * N1 - L) - (1.0 + 0x1p-20) * k * k;
     W[s] = __double2float_rd(w);
+    N2[s] = N;
+    if (s == 0) sigOut[0] = sigma;
 }
 
 template <typename T, int LP>
@@ -139,6 +143,7 @@ static int prepare_t(GramData &gd, const Grid &G, hipStream_t s) {
     GDSJ_TRY(gd.h.ensure((size_t)n * dpad * 2 + 256));
     if (gd.split) GDSJ_TRY(gd.l.ensure((size_t)n * dpad * 2 + 256));
     GDSJ_TRY(gd.w.ensure((size_t)n * 4 + 256));
+    GDSJ_TRY(gd.n2.ensure((size_t)n * 8 + 256));
     GDSJ_TRY(gd.scratch.ensure((size_t)d * 16 + 256));
     double *mean = gd.scratch.as<double>();
     double *maxdev = mean + d;
@@ -153,7 +158,8 @@ static int prepare_t(GramData &gd, const Grid &G, hipStream_t s) {
     const double cfac = (2.0 * (gd.split ? 3.0 : 1.0) * dpad + 16.0) * 0x1p-24;
     k_gram_convert<T, LP><<<ceil_div(n, 256), 256, 0, s>>>(
         xs, G.ld, n, d, dpad, mean, maxdev, eps_p, cfac, gd.split, (typename Lp<LP>::scalar *)gd.h.p,
-        gd.split ? (typename Lp<LP>::scalar *)gd.l.p : nullptr, gd.w.as<float>());
+        gd.split ? (typename Lp<LP>::scalar *)gd.l.p : nullptr, gd.w.as<float>(), gd.n2.as<double>(),
+        mean + 2 * (size_t)d);
     GDSJ_HIP(hipGetLastError());
     return GDSJ_OK;
 }
@@ -173,19 +179,19 @@ int gram_prepare(GramData &gd, const Grid &G, int lowp, hipStream_t s) {
 }
 
 int gram_dispatch_f32(const GramData &gd, const JoinArgs &a, int mode, int pass, unsigned long long *stats,
-                      hipStream_t s);
+                      bool err, hipStream_t s);
 int gram_dispatch_f64(const GramData &gd, const JoinArgs &a, int mode, int pass, unsigned long long *stats,
-                      hipStream_t s);
+                      bool err, hipStream_t s);
 
 int gram_join_pass(const Grid &G, const GramData &gd, const JoinArgs &a, int mode, int pass,
-                   unsigned long long *stats, hipStream_t s) {
+                   unsigned long long *stats, hipStream_t s, bool measure_err) {
     if (G.tileq > gram_tile_queries(G.d, gd.lowp)) {
         set_error("Gram tiles hold at most %u queries here", gram_tile_queries(G.d, gd.lowp));
         return GDSJ_EINVAL;
     }
     // FP32 data uses bf16 (split in two terms), FP64 data fp16
-    if (G.dtype == GDSJ_DTYPE_F32) return gram_dispatch_f32(gd, a, mode, pass, stats, s);
-    return gram_dispatch_f64(gd, a, mode, pass, stats, s);
+    if (G.dtype == GDSJ_DTYPE_F32) return gram_dispatch_f32(gd, a, mode, pass, stats, measure_err, s);
+    return gram_dispatch_f64(gd, a, mode, pass, stats, measure_err, s);
 }
 
 }  // namespace gdsj

@@ -4,8 +4,8 @@
 namespace gdsj {
 
 int gram_dispatch_f32(const GramData &gd, const JoinArgs &a, int mode, int pass, unsigned long long *stats,
-                      hipStream_t s) {
-    return gram_dispatch<float, 0>(gd, a, mode, pass, stats, s);
+                      bool err, hipStream_t s) {
+    return gram_dispatch<float, 0>(gd, a, mode, pass, stats, err, s);
 }
 
 }  // namespace gdsj

@@ -43,13 +43,25 @@ constexpr int rb_max(int ks, int split) {
     return r < 1 ? 1 : (r > 4 ? 4 : r);
 }
 
+// Optional measurement of the filter's error on the rechecked pairs (a
+// separate counting-pass variant, ERR, so the timed kernels keep their
+// register allocation): the
+// Gram estimate N_q + N_p - 2 S of the scaled squared distance against the
+// squared distance in FP64, max of |difference| / (sigma eps)^2 into stats[2]
+// (the bits of a non-negative double, ordered as an integer).
+struct GramAux {
+    const double *nrm;  // N = |h + l|^2 per point (scaled units)
+    const double *sig;  // sigma
+    uint32_t err;       // 1: measure
+};
+
 // ---------------------------------------------------------------- the join --
-template <typename T, int LP, int KS, int SPLIT, int PASS>
+template <typename T, int LP, int KS, int SPLIT, int PASS, bool ERR>
 __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
                                               const typename Lp<LP>::scalar *__restrict__ H,
                                               const typename Lp<LP>::scalar *__restrict__ Lo,
                                               const float *__restrict__ W, const JoinArgs a, int mode,
-                                              unsigned long long *__restrict__ stats) {
+                                              unsigned long long *__restrict__ stats, const GramAux aux) {
     using vec = typename Lp<LP>::vec;
     constexpr uint32_t DP = KS * 16;
     constexpr int RB = rb_max(KS, SPLIT);
@@ -131,6 +143,7 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     }
     uint64_t work = 0;
     uint32_t nrech = 0, nhit = 0;
+    double errmax = 0.0;
     bool overflow = false;
     static_assert(TQ <= 128, "rows are owned by lanes l and l + 64");
 
@@ -183,6 +196,21 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
 #pragma unroll
                 for (int i = 0; i < 16; i++) sm |= (acc[i] >= wq[rb][i] + wc) ? (1u << i) : 0u;
                 if (ballot(sm != 0) == 0) continue;
+                if (ERR && sm) {
+                    const double s2 = aux.sig[0] * aux.sig[0];
+#pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        if (!((sm >> i) & 1u)) continue;
+                        const uint32_t r = rb * 32 + (i / 4) * 8 + half * 4 + (i % 4);
+                        const double est = aux.nrm[q0 + r] + aux.nrm[sp] - 2.0 * (double)acc[i];
+                        double ex = 0.0;
+                        for (uint32_t l = 0; l < a.d; l++) {
+                            const double t = (double)xs[(uint64_t)l * ld + sp] - (double)xs[(uint64_t)l * ld + q0 + r];
+                            ex += t * t;
+                        }
+                        errmax = fmax(errmax, fabs(est - s2 * ex) / (s2 * a.ga.eps * a.ga.eps));
+                    }
+                }
                 // exact rechecks, lane-divergent; hits set column bits of their row
 #pragma unroll 1
                 while (sm) {
@@ -274,12 +302,17 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
             atomicAdd(&stats[0], r1);
             atomicAdd(&stats[1], r2);
         }
+        if (ERR) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) errmax = fmax(errmax, __shfl_xor(errmax, off, GDSJ_WAVE));
+            if (lane == 0 && errmax > 0.0) atomicMax(&stats[2], (unsigned long long)__double_as_longlong(errmax));
+        }
     }
 }
 
 template <typename T, int LP, int KS>
 inline int launch_gram(const GramData &gd, const JoinArgs &a, int mode, int pass, unsigned long long *stats,
-                       hipStream_t s) {
+                       bool err, hipStream_t s) {
     const uint32_t blocks = (a.ntiles + 3) / 4;
     if (blocks == 0) return GDSJ_OK;
     const T *xs = (const T *)a.xs;
@@ -287,26 +320,29 @@ inline int launch_gram(const GramData &gd, const JoinArgs &a, int mode, int pass
     const auto *Lo = (const typename Lp<LP>::scalar *)(gd.split ? gd.l.p : gd.h.p);
     const float *W = gd.w.as<float>();
     constexpr int SP = LP == 0 ? 1 : 0;
+    const GramAux aux{gd.n2.as<double>(), gd.scratch.as<double>() + 2 * (size_t)a.d, err && stats ? 1u : 0u};
     if (pass == PASS_FILL)
-        k_gram<T, LP, KS, SP, PASS_FILL><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats);
+        k_gram<T, LP, KS, SP, PASS_FILL, false><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats, aux);
     else if (pass == PASS_SLOT)
-        k_gram<T, LP, KS, SP, PASS_SLOT><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats);
+        k_gram<T, LP, KS, SP, PASS_SLOT, false><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats, aux);
+    else if (aux.err)  // measurement variant (counting pass only)
+        k_gram<T, LP, KS, SP, PASS_COUNT, true><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats, aux);
     else
-        k_gram<T, LP, KS, SP, PASS_COUNT><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats);
+        k_gram<T, LP, KS, SP, PASS_COUNT, false><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats, aux);
     GDSJ_HIP(hipGetLastError());
     return GDSJ_OK;
 }
 
 template <typename T, int LP>
 inline int gram_dispatch(const GramData &gd, const JoinArgs &a, int mode, int pass, unsigned long long *stats,
-                         hipStream_t s) {
+                         bool err, hipStream_t s) {
     switch (gd.dpad / 16) {
-        case 1: return launch_gram<T, LP, 1>(gd, a, mode, pass, stats, s);
-        case 2: return launch_gram<T, LP, 2>(gd, a, mode, pass, stats, s);
-        case 3: return launch_gram<T, LP, 3>(gd, a, mode, pass, stats, s);
-        case 4: return launch_gram<T, LP, 4>(gd, a, mode, pass, stats, s);
-        case 6: return launch_gram<T, LP, 6>(gd, a, mode, pass, stats, s);
-        case 8: return launch_gram<T, LP, 8>(gd, a, mode, pass, stats, s);
+        case 1: return launch_gram<T, LP, 1>(gd, a, mode, pass, stats, err, s);
+        case 2: return launch_gram<T, LP, 2>(gd, a, mode, pass, stats, err, s);
+        case 3: return launch_gram<T, LP, 3>(gd, a, mode, pass, stats, err, s);
+        case 4: return launch_gram<T, LP, 4>(gd, a, mode, pass, stats, err, s);
+        case 6: return launch_gram<T, LP, 6>(gd, a, mode, pass, stats, err, s);
+        case 8: return launch_gram<T, LP, 8>(gd, a, mode, pass, stats, err, s);
         default: break;
     }
     set_error("unsupported padded dims %u for the Gram filter", gd.dpad);

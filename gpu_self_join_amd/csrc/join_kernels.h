@@ -26,6 +26,17 @@ struct CMaxFor {
     static constexpr int value = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
 };
 
+// value of lane i (wave-uniform i) as a scalar
+__device__ __forceinline__ float lane_value(float v, uint32_t i) {
+    return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), i));
+}
+__device__ __forceinline__ double lane_value(double v, uint32_t i) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, i);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), i);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Squared distance in the reference's order.  q: query coords (uniform),
 // c: candidate coords (this lane).  Returns false early (acc > thr on every
 // lane of the wave that is still a candidate) when short-circuiting; the
@@ -165,6 +176,14 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
         }
     }
     const uint32_t qid_l = lane < nq ? ids[q0 + lane] : 0u;
+    // low dims: lane l holds query l's coordinates; the query loop reads them
+    // with readlane (no memory access per query)
+    constexpr bool QREG = D > 0 && D * (int)(sizeof(T) / 4) <= 16;
+    T qv[QREG ? D : 1];
+    if (QREG) {
+#pragma unroll
+        for (int l = 0; l < (QREG ? D : 1); l++) qv[l] = lane < nq ? xs[(uint64_t)l * ld + q0 + lane] : (T)0;
+    }
     uint32_t cnt_l = 0;
     uint64_t cur_l = 0;
     uint64_t work = 0;
@@ -222,7 +241,10 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
             for (uint32_t qi = 0; qi < nq; qi++) {
                 const uint32_t qpos = q0 + qi;
                 T q[D > 0 ? D : 1];
-                if (D > 0) {
+                if (QREG) {
+#pragma unroll
+                    for (int l = 0; l < (D > 0 ? D : 1); l++) q[l] = lane_value(qv[l], qi);
+                } else if (D > 0) {
 #pragma unroll
                     for (int l = 0; l < (D > 0 ? D : 1); l++) q[l] = xs[(uint64_t)l * ld + qpos];
                 }

@@ -71,7 +71,9 @@ typedef struct gdsj_params {
                                0 auto, 1 on, 2 off (speed only) */
     uint32_t train;         /* learned estimator: 0 train until first trained, 1 after every
                                join, 2 never */
-    uint32_t reserved[2];
+    uint32_t filter_stats;  /* 1: the Gram filter also measures its error on the rechecked
+                               pairs (gdsj_stats.filter_max_err; slower) */
+    uint32_t reserved;
 } gdsj_params;
 
 /* Timing and diagnostics of the last join on a handle. */
@@ -90,7 +92,8 @@ typedef struct gdsj_stats {
     uint64_t est_tiles;          /* tiles sized by the estimator */
     double est_mean_rel_err;     /* mean |predicted - actual| / actual over tiles */
     double est_max_rel_err;
-    double filter_max_err;       /* reserved */
+    double filter_max_err;       /* Gram filter, with params.filter_stats: max over rechecked pairs
+                                    of |Gram-form squared distance - FP64 squared distance| / eps^2 */
     double ms_grid;              /* reorder + grid + neighbor runs (device) */
     double ms_estimate;          /* estimator inference + slot sizing */
     double ms_count;             /* counting (or slotted) join pass */
