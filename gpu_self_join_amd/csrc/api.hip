@@ -51,6 +51,8 @@ struct gdsj_handle {
     GramData gram;
     DevBuf gramStats;
     bool use_gram = false;
+    bool stamp = false;  // unicomp in force for this join
+    DevBuf fcnt, rcur;   // unicomp: own-side counts, reverse-entry cursors
     bool slotted = false;
     uint32_t novf = 0;
     gdsj_params p{};
@@ -145,6 +147,8 @@ int gdsj_destroy(gdsj_handle *h) {
     (void)hipSetDevice(h->device);
     if (h->own) (void)hipStreamSynchronize(h->own);
     h->G.release();
+    h->fcnt.release();
+    h->rcur.release();
     DevBuf *b[] = {&h->counts,  &h->offsets,  &h->ids,     &h->data,    &h->scanTmp, &h->status,
                    &h->work,    &h->slotCap,  &h->slotOff, &h->slotUsed, &h->slotOvf, &h->slotIds,
                    &h->slotQ,   &h->scalars,  &h->estPred, &h->estScratch};
@@ -202,6 +206,11 @@ static JoinArgs make_args(gdsj_handle *h) {
     // clustered data); query-box pruning from k >= 4, where it pays
     a.xcd_map = h->p.schedule == 2 ? 1u : 0u;
     a.prune = h->p.prune == 1 || (h->p.prune == 0 && h->G.k >= 4) ? 1u : 0u;
+    if (h->stamp) {
+        a.stamp = 1;
+        a.fcounts = h->fcnt.as<uint32_t>();
+        a.rcur = h->rcur.as<uint32_t>();
+    }
     if (h->G.haveRuns) {
         a.runs = h->G.runs.as<uint2>();
         a.nruns = h->G.nruns.as<uint32_t>();
@@ -240,6 +249,7 @@ static int do_fill(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
     a.cap = cap;
     GDSJ_HIP(hipMemsetAsync(h->status.p, 0, 4, s));
     GDSJ_HIP(hipEventRecord(h->ev[4], s));
+    if (h->stamp && h->G.n) GDSJ_HIP(hipMemsetAsync(h->rcur.p, 0, (size_t)h->G.n * 4, s));
     if (h->slotted) {
         set_slot_args(h, a);
         GDSJ_TRY(compact_slots(a, s));
@@ -355,6 +365,14 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
         set_error("the Gram filter handles 16 to 128 dims (dims=%u)", dims);
         return GDSJ_EUNSUPPORTED;
     }
+    // unicomp (STAMP) with the plain kernels and the exact passes; with the
+    // Gram filter or an estimator the full neighborhood is evaluated instead
+    // (the result is the same either way)
+    h->stamp = h->p.stamp && !h->use_gram && h->p.estimator == GDSJ_EST_EXACT;
+    if (h->stamp) {
+        GDSJ_TRY(h->fcnt.ensure((size_t)n * 4 + 256));
+        GDSJ_TRY(h->rcur.ensure((size_t)n * 4 + 256));
+    }
     uint32_t tq = h->p.tile_queries ? h->p.tile_queries : 64;
     if (h->use_gram) {
         const uint32_t gq = gram_tile_queries(dims, dtype == GDSJ_DTYPE_F64 ? 1 : 0);
@@ -371,7 +389,7 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     {
         bool built = false;
         const bool prune = h->p.prune == 1 || (h->p.prune == 0 && h->G.k >= 4);
-        GDSJ_TRY(build_runs(h->G, s, prune, &built));
+        GDSJ_TRY(build_runs(h->G, s, prune, h->stamp, &built));
         h->G.haveRuns = built;
     }
     uint64_t *offs = d_offsets;
@@ -395,6 +413,7 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     if (!slotted) {
         GDSJ_HIP(hipEventRecord(h->ev[1], s));
         GDSJ_HIP(hipEventRecord(h->ev[2], s));
+        if (h->stamp && n) GDSJ_HIP(hipMemsetAsync(d_counts, 0, (size_t)n * 4, s));  // accumulated by atomics
         GDSJ_TRY(run_pass(h, a, PASS_COUNT));
     }
     h->slotted = slotted;

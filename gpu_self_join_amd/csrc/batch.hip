@@ -133,7 +133,7 @@ int build_all(gdsj_batch *b, uint32_t tq) {
         GDSJ_TRY(grid_build(g, x.d_data, x.n, x.dims, b->dtype, x.epsilon, b->k[i], b->p.reorder, tq, 0, s));
         bool built = false;
         const bool prune = b->p.prune == 1 || (b->p.prune == 0 && g.k >= 4);
-        GDSJ_TRY(build_runs(g, s, prune, &built));
+        GDSJ_TRY(build_runs(g, s, prune, false, &built));
         g.haveRuns = built;
         if (!built && g.ntiles) b->all_runs = false;
         const uint64_t next = (uint64_t)b->tileBase[i] + g.ntiles;

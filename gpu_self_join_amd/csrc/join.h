@@ -54,6 +54,13 @@ struct JoinArgs {
     uint32_t *ovfList;        // tiles whose slot overflowed (tileBase + tile)
     uint32_t *ovfCount;
     uint32_t tileBase;        // first global tile id of this join in a batch (0 alone)
+    // unicomp (STAMP): each adjacent cell pair is evaluated from its owner's
+    // tiles only and both directions are emitted; counts[] is accumulated
+    // with atomics (zeroed before the counting pass), fcounts[q] = the
+    // query's own-side count, rcur[j] = reverse entries placed so far
+    uint32_t stamp;
+    uint32_t *fcounts;
+    uint32_t *rcur;
     uint32_t *status;  // [0] set when a fill write would exceed cap
 };
 
@@ -63,7 +70,8 @@ int join_pass(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t 
 // Builds the compacted neighbor-run list of every cell (runs, nruns,
 // cellCand = candidates per query of the cell) when it fits the memory cap;
 // returns false (and leaves the lists unset) otherwise.
-int build_runs(Grid &G, hipStream_t s, bool prune, bool *built);
+// stamp: only the neighbor cells each cell owns (unicomp).
+int build_runs(Grid &G, hipStream_t s, bool prune, bool stamp, bool *built);
 
 // Moves every non-overflowed tile's slot entries into CSR rows.
 int compact_slots(const JoinArgs &a, hipStream_t s);

@@ -10,12 +10,19 @@ static int launch_join_t(const Grid &G, const JoinArgs &a, int pass, hipStream_t
     const uint32_t blocks = (a.ntiles + waves - 1) / waves;
     if (blocks == 0) return GDSJ_OK;
     const JT *xs = (const JT *)a.xs;
-    if (pass == PASS_FILL)
-        k_join<JT, D, MODE, ILPV, PASS_FILL><<<blocks, 256, 0, s>>>(xs, a.ids, a);
-    else if (pass == PASS_SLOT)
+    if (a.stamp && pass == PASS_SLOT) {
+        set_error("unicomp runs with the counting and fill passes only");
+        return GDSJ_EINVAL;
+    }
+    if (pass == PASS_FILL) {
+        if (a.stamp) k_join<JT, D, MODE, ILPV, PASS_FILL, true><<<blocks, 256, 0, s>>>(xs, a.ids, a);
+        else k_join<JT, D, MODE, ILPV, PASS_FILL><<<blocks, 256, 0, s>>>(xs, a.ids, a);
+    } else if (pass == PASS_SLOT) {
         k_join<JT, D, MODE, ILPV, PASS_SLOT><<<blocks, 256, 0, s>>>(xs, a.ids, a);
-    else
-        k_join<JT, D, MODE, ILPV, PASS_COUNT><<<blocks, 256, 0, s>>>(xs, a.ids, a);
+    } else {
+        if (a.stamp) k_join<JT, D, MODE, ILPV, PASS_COUNT, true><<<blocks, 256, 0, s>>>(xs, a.ids, a);
+        else k_join<JT, D, MODE, ILPV, PASS_COUNT><<<blocks, 256, 0, s>>>(xs, a.ids, a);
+    }
     GDSJ_HIP(hipGetLastError());
     return GDSJ_OK;
 }

@@ -143,11 +143,20 @@ __device__ __forceinline__ T dist_mem(const T *__restrict__ xs, uint64_t ld, uin
 // One work tile of the join (one wave): the body shared by the per-join
 // kernel below and the persistent batched kernel (batch.hip).
 // CM: candidate chunks held in VGPRs (0: CMaxFor's default).
-template <typename T, int D, int MODE, int ILPV, int PASS, int CM = 0>
+// STAMP (unicomp; count and fill passes): the tile's candidates are its own
+// cell and the neighbor cells it owns.  A hit (q, p) with p outside the own
+// cell is also the hit (p, q), which no other tile evaluates: the counting
+// pass adds each candidate's hits over the tile's queries to counts[p] (one
+// atomic per candidate and chunk batch), the fill pass collects the hitting
+// queries of each candidate in a 64-bit mask and places them behind p's own
+// entries with one atomic on p's cursor (their order within a row depends
+// on the order the tiles run).
+template <typename T, int D, int MODE, int ILPV, int PASS, int CM = 0, bool STAMP = false>
 __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
                                           const JoinArgs &a, const uint32_t tile) {
     constexpr int CMAX = CM > 0 ? CM : CMaxFor<T, D>::value;
-    constexpr bool NEED_IDS = PASS != PASS_COUNT;
+    constexpr bool NEED_IDS = PASS != PASS_COUNT || STAMP;
+    static_assert(!STAMP || PASS != PASS_SLOT, "unicomp runs with the counting and fill passes");
     const uint32_t lane = lane_id();
     const uint64_t ld = a.ld;
     const T thr = (T)a.thr;
@@ -176,6 +185,14 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
         }
     }
     const uint32_t qid_l = lane < nq ? ids[q0 + lane] : 0u;
+    __shared__ uint32_t s_qid[STAMP ? 256 / GDSJ_WAVE : 1][GDSJ_WAVE];
+    uint32_t own0 = 0, own1 = 0;  // sorted-position range of the tile's own cell
+    if (STAMP) {
+        own0 = uniform(a.cellStart[cell]);
+        own1 = uniform(a.cellStart[cell + 1]);
+        s_qid[threadIdx.x / GDSJ_WAVE][lane] = qid_l;
+        __builtin_amdgcn_wave_barrier();
+    }
     // low dims: lane l holds query l's coordinates; the query loop reads them
     // with readlane (no memory access per query)
     constexpr bool QREG = D > 0 && D * (int)(sizeof(T) / 4) <= 16;
@@ -201,7 +218,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
         uint2 run = make_uint2(0u, 0u);
         if (lane < nrw)
             run = pre ? a.runs[(uint64_t)cell * a.runStride + cw + lane]
-                      : combo_run(cw + lane, cc, glo, ghi, ga, a.cellStart, a.hkeys, a.hvals);
+                      : combo_run(cw + lane, cc, glo, ghi, ga, a.cellStart, a.hkeys, a.hvals, STAMP);
         uint32_t len = run.y - run.x;
         uint32_t incl = len;
 #pragma unroll
@@ -225,13 +242,21 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
             uint32_t cpos[CMAX];
             uint32_t cid[CMAX];
             bool valid[CMAX];
+            uint64_t vmask[CMAX];
+            bool other[CMAX];                                    // STAMP: candidate outside the own cell
+            uint32_t jc[STAMP && PASS == PASS_COUNT ? CMAX : 1];  // STAMP: hits of this candidate
+            uint64_t jm[STAMP && PASS == PASS_FILL ? CMAX : 1];   // STAMP: queries hitting it
 #pragma unroll
             for (int ch = 0; ch < CMAX; ch++) {
                 const uint32_t g = cb + ch * GDSJ_WAVE + lane;
                 valid[ch] = g < total;
+                vmask[ch] = ballot(valid[ch]);
+                if (STAMP && PASS == PASS_COUNT) jc[ch] = 0;
+                if (STAMP && PASS == PASS_FILL) jm[ch] = 0;
                 uint32_t sp = cand_pos(valid[ch] ? g : 0u, nrw, rstart, rexcl);
                 sp = valid[ch] ? sp : fpos;
                 cpos[ch] = sp;
+                other[ch] = STAMP && valid[ch] && (sp < own0 || sp >= own1);
                 if (D > 0 && ch < (int)nch) {
 #pragma unroll
                     for (int l = 0; l < (D > 0 ? D : 1); l++) c[ch][l] = xs[(uint64_t)l * ld + sp];
@@ -250,31 +275,43 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                 }
                 uint32_t tot = 0;
                 uint64_t cur = 0;
+                uint32_t *orow = nullptr;  // this query's next CSR slot (wave-uniform)
+                uint32_t room = 0;         // slots left before the output capacity
                 if (PASS == PASS_FILL) {
                     uint32_t lo = __builtin_amdgcn_readlane((uint32_t)cur_l, qi);
                     uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(cur_l >> 32), qi);
                     cur = ((uint64_t)hi << 32) | lo;
+                    orow = a.out + cur;
+                    const uint64_t left = cur < a.cap ? a.cap - cur : 0ull;
+                    room = left > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)left;
                 }
 #pragma unroll
                 for (int ch = 0; ch < CMAX; ch++) {
                     if (ch < (int)nch) {
+                        // the compare's own lane mask, restricted to valid lanes
+                        uint64_t m;
                         bool hit;
                         if (D > 0) {
                             bool alive;
                             T acc = dist_reg<T, (D > 0 ? D : 1), MODE, (ILPV > 0 ? ILPV : 1)>(
                                 q, c[ch], thr, valid[ch], a.sc != 0, alive);
                             hit = alive && valid[ch] && acc <= thr;
+                            m = alive ? (ballot(acc <= thr) & vmask[ch]) : 0ull;
                         } else {
                             T acc = dist_mem<T>(xs, ld, qpos, cpos[ch], a.d, MODE, a.ilp);
                             hit = valid[ch] && acc <= thr;
+                            m = ballot(acc <= thr) & vmask[ch];
                         }
-                        const uint64_t m = ballot(hit);
                         const uint32_t pm = popc64(m);
+                        if (STAMP && PASS == PASS_COUNT && hit) jc[ch]++;
+                        if (STAMP && PASS == PASS_FILL && hit) jm[ch] |= 1ull << qi;
                         if (PASS == PASS_FILL) {
                             if (hit) {
-                                const uint64_t pos = cur + mbcnt(m);
-                                if (pos < a.cap) a.out[pos] = cid[ch]; else overflow = true;
+                                const uint32_t r = mbcnt(m);
+                                if (r < room) orow[r] = cid[ch]; else overflow = true;
                             }
+                            orow += pm;
+                            room = room > pm ? room - pm : 0u;
                             cur += pm;
                         } else {
                             if (PASS == PASS_SLOT && pm) {
@@ -297,9 +334,40 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                     if (lane == qi) cnt_l += tot;
                 }
             }
+            if (STAMP) {
+                // the reverse direction of this chunk batch's hits
+#pragma unroll
+                for (int ch = 0; ch < CMAX; ch++) {
+                    if (ch >= (int)nch || !other[ch]) continue;
+                    if (PASS == PASS_COUNT) {
+                        if (jc[ch]) atomicAdd(&a.counts[cid[ch]], jc[ch]);
+                    } else if (PASS == PASS_FILL) {
+                        uint64_t mm = jm[ch];
+                        if (mm) {
+                            const uint32_t p = cid[ch];
+                            const uint32_t b = atomicAdd(&a.rcur[p], (uint32_t)popc64(mm));
+                            uint64_t pos = a.offsets[p] + a.fcounts[p] + b;
+                            while (mm) {
+                                const int bit = __builtin_ctzll(mm);
+                                mm &= mm - 1;
+                                if (pos < a.cap) a.out[pos] = s_qid[threadIdx.x / GDSJ_WAVE][bit];
+                                else overflow = true;
+                                pos++;
+                            }
+                        }
+                    }
+                }
+            }
         }
     }
-    if (PASS != PASS_FILL && lane < nq) a.counts[qid_l] = cnt_l;
+    if (PASS != PASS_FILL && lane < nq) {
+        if (STAMP) {
+            a.fcounts[qid_l] = cnt_l;
+            atomicAdd(&a.counts[qid_l], cnt_l);
+        } else {
+            a.counts[qid_l] = cnt_l;
+        }
+    }
     if (PASS != PASS_FILL && a.tileWork && lane == 0) a.tileWork[tile] = work * nq;
     if (PASS == PASS_SLOT && lane == 0) {
         a.slotUsed[tile] = scur;
@@ -308,14 +376,14 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
     if (PASS == PASS_FILL && overflow) atomicOr(a.status, 1u);
 }
 
-template <typename T, int D, int MODE, int ILPV, int PASS>
+template <typename T, int D, int MODE, int ILPV, int PASS, bool STAMP = false>
 __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
                                               const JoinArgs a) {
     const uint32_t lblock = a.xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t tslot = uniform((uint32_t)(lblock * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
     if (tslot >= a.ntiles) return;
     const uint32_t tile = a.tileList ? uniform(a.tileList[tslot]) : tslot;
-    join_tile<T, D, MODE, ILPV, PASS>(xs, ids, a, tile);
+    join_tile<T, D, MODE, ILPV, PASS, 0, STAMP>(xs, ids, a, tile);
 }
 
 }  // namespace gdsj

@@ -30,15 +30,23 @@ __device__ __forceinline__ uint32_t cand_pos(uint32_t g, uint32_t nrw, uint32_t 
 // [0, nCells-1] are skipped, as the reference's clamps do (kernel.cu:183-187).
 // Cells whose squared gap to the tile's query box exceeds prune_lim are
 // skipped too: no pair with them can pass the predicate.
+//
+// stamp (unicomp, stamploops.h:2-201): keep only the cells this cell owns.
+// An adjacent pair of cells is owned by the one whose coordinate is odd in
+// the highest dimension where they differ, so each unordered pair is
+// evaluated from one side only: a row of dims 1..k-1 whose highest nonzero
+// offset is in dim j is kept iff c_j is odd; the cell's own row keeps the
+// dim-0 neighbors iff c_0 is odd (the own cell always).
 __device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK], const double *glo,
                                            const double *ghi, const GridArgs &ga,
                                            const uint32_t *__restrict__ cellStart,
                                            const uint64_t *__restrict__ hkeys,
-                                           const uint32_t *__restrict__ hvals) {
+                                           const uint32_t *__restrict__ hvals, bool stamp = false) {
     uint64_t base = 0;
     bool ok = true;
     double gap = 0.0;
     uint32_t tt = t;
+    int top = -1;  // highest dim (>= 1) with a nonzero offset
     for (uint32_t j = 1; j < ga.k; j++) {
         const uint32_t dlt = tt % 3u;
         tt /= 3u;
@@ -46,13 +54,16 @@ __device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK
         ok &= (cj >= 0) && (cj < (int64_t)ga.nCells[j]);
         gap += dlt == 0 ? glo[j] : (dlt == 2 ? ghi[j] : 0.0);
         base += (uint64_t)cj * ga.mult[j];
+        if (dlt != 1) top = (int)j;
     }
     uint2 run = make_uint2(0u, 0u);
     if (!ok || gap > ga.prune_lim) return run;
+    if (stamp && top >= 0 && !(cc[top] & 1u)) return run;
     uint32_t x0 = cc[0] == 0 ? 0u : cc[0] - 1u;
     uint32_t x1 = cc[0] + 1u < ga.nCells[0] ? cc[0] + 1u : ga.nCells[0] - 1u;
     if (x0 < cc[0] && gap + glo[0] > ga.prune_lim) x0 = cc[0];
     if (x1 > cc[0] && gap + ghi[0] > ga.prune_lim) x1 = cc[0];
+    if (stamp && top < 0 && !(cc[0] & 1u)) x0 = x1 = cc[0];
     int32_t first = -1, last = -1;
     for (uint32_t x = x0; x <= x1; x++) {
         const int32_t ci = cell_find(hkeys, hvals, ga.hbits, base + x);

@@ -20,7 +20,8 @@ __global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uin
                                                     const uint32_t *__restrict__ cellCoord,
                                                     const uint64_t *__restrict__ hkeys,
                                                     const uint32_t *__restrict__ hvals, uint32_t ncombo,
-                                                    uint64_t stride, int prune, uint2 *__restrict__ runs,
+                                                    uint64_t stride, int prune, int stamp,
+                                                    uint2 *__restrict__ runs,
                                                     uint32_t *__restrict__ nruns, uint32_t *__restrict__ cellCand,
                                                     float *__restrict__ cellHist) {
     const uint32_t lane = lane_id();
@@ -47,7 +48,8 @@ __global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uin
     uint2 *mine = runs + (uint64_t)cell * stride;
     for (uint32_t cw = 0; cw < ncombo; cw += GDSJ_WAVE) {
         uint2 run = make_uint2(0u, 0u);
-        if (cw + lane < ncombo) run = combo_run(cw + lane, cc, glo, ghi, ga, cellStart, hkeys, hvals);
+        if (cw + lane < ncombo)
+            run = combo_run(cw + lane, cc, glo, ghi, ga, cellStart, hkeys, hvals, stamp != 0);
         const bool ne = run.y > run.x;
         const uint64_t m = ballot(ne);
         if (ne) mine[cursor + mbcnt(m)] = run;
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uin
     }
 }
 
-int build_runs(Grid &G, hipStream_t s, bool prune, bool *built) {
+int build_runs(Grid &G, hipStream_t s, bool prune, bool stamp, bool *built) {
     *built = false;
     uint64_t ncombo = 1;
     for (uint32_t j = 1; j < G.k; j++) ncombo *= 3;
@@ -86,13 +88,14 @@ int build_runs(Grid &G, hipStream_t s, bool prune, bool *built) {
         k_cell_runs<float><<<blocks, 256, 0, s>>>(G.xs.as<float>(), G.ld, G.ga, G.cellStart.as<uint32_t>(),
                                                   G.cellCoord.as<uint32_t>(), G.hkeys.as<uint64_t>(),
                                                   G.hvals.as<uint32_t>(), (uint32_t)ncombo, ncombo, prune ? 1 : 0,
-                                                  G.runs.as<uint2>(), G.nruns.as<uint32_t>(),
+                                                  stamp ? 1 : 0, G.runs.as<uint2>(), G.nruns.as<uint32_t>(),
                                                   G.cellCand.as<uint32_t>(), G.cellHist.as<float>());
     else
         k_cell_runs<double><<<blocks, 256, 0, s>>>(G.xs.as<double>(), G.ld, G.ga, G.cellStart.as<uint32_t>(),
                                                    G.cellCoord.as<uint32_t>(), G.hkeys.as<uint64_t>(),
                                                    G.hvals.as<uint32_t>(), (uint32_t)ncombo, ncombo,
-                                                   prune ? 1 : 0, G.runs.as<uint2>(), G.nruns.as<uint32_t>(),
+                                                   prune ? 1 : 0, stamp ? 1 : 0, G.runs.as<uint2>(),
+                                                   G.nruns.as<uint32_t>(),
                                                    G.cellCand.as<uint32_t>(), G.cellHist.as<float>());
     GDSJ_HIP(hipGetLastError());
     *built = true;

@@ -18,6 +18,7 @@ import pytest
 
 from gpu_self_join_amd import _lib, datagen
 from oracle import oracle as O
+from tests.helpers import sorted_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -79,7 +80,13 @@ def test_full_size_table_equals_oracle(gpu, name):
             _run(sl, pts, eps, estimator=_lib.EST_LEARNED, **kw)  # exact + training
             c2, o2, i2, r2 = _run(sl, pts, eps, estimator=_lib.EST_LEARNED, **kw)
             assert r2.stats["est_tiles"] > 0
-            assert np.array_equal(c2, counts) and np.array_equal(i2, ids)
+            assert np.array_equal(c2, counts)
+            if p["stamp"]:
+                # unicomp (exact passes) orders the reverse entries of a row by
+                # tile completion; the slotted pass evaluates both directions
+                assert np.array_equal(sorted_rows(c2, i2), sorted_rows(counts, ids))
+            else:
+                assert np.array_equal(i2, ids)
         finally:
             sl.close()
     finally:
