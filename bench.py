@@ -89,7 +89,7 @@ def cpu_reference_paths(x, eps, budget_s=8.0, seed=321):
     return out
 
 
-EST = {"exact": 0, "learned": 1, "sample": 2, "plain": 0}
+EST = {"exact": 0, "learned": 1, "sample": 2, "plain": 0, "nostamp": 0}
 
 
 def copy_probe(nbytes, reps=5):
@@ -177,7 +177,7 @@ def bench_batched(args):
     coord_bytes = sum(x.size * 4 for x in host)
     results = {}
     total = None
-    for vname in args.variants.split(","):
+    for vname in (args.variants or "exact,learned").split(","):
         bj = BatchJoin(xs, eps, estimator=EST[vname])
         tune = bj.autotune()
         bj.run()  # capture (learned: exact run + training)
@@ -251,7 +251,7 @@ def main():
     ap.add_argument("--workload", default="gmm6d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=float, default=0.01)
-    ap.add_argument("--variants", default="exact,learned",
+    ap.add_argument("--variants", default=None,
                     help="estimators to time (exact = counting pass + fill; learned = MLP-sized single pass)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (HBM counter calibration)")
@@ -288,6 +288,8 @@ def main():
             raise SystemExit("--param %s would change the workload's result contract" % key)
         base[key] = int(val)
     esz = np.dtype(w["dtype"]).itemsize
+    if args.variants is None:
+        args.variants = "exact,nostamp,learned" if base.get("stamp") else "exact,learned"
     results = {}
     total = None
     for vname in args.variants.split(","):
@@ -296,6 +298,10 @@ def main():
         steps, warmup = args.steps, args.warmup
         if vname == "learned":
             kw["train"] = 0  # trains on the first join's counts, then sizes slots
+        if vname == "nostamp":
+            # the exact passes over the full neighborhood (STAMP only changes
+            # which side evaluates a pair, not the table)
+            kw["stamp"] = 0
         if vname == "plain":
             # the plain grid join without the MFMA filter (the reference's
             # method on this hardware): few steps, it is the slow reference point
@@ -316,7 +322,7 @@ def main():
     kb = {"count": n * d * esz + 4 * n, "fill": n * d * esz + 4 * total}
     kms = {"count": phase["ms_count"], "fill": phase["ms_fill"]}
     kname = {"count": "k_join<count pass>", "fill": "k_join<fill pass>"}
-    if best != "exact":
+    if best in ("learned", "sample"):
         kb["count"] = n * d * esz + 4 * n + 5 * total
         kname["count"] = "k_join<slotted pass>"
         kb["compact"] = 9 * total
@@ -345,7 +351,7 @@ def main():
                            "phase_ms": {k: round(v, 4) for k, v in vr["phase_ms"].items()}}
         variants[vname]["filter"] = {"rechecks": st["filter_rechecks"], "hits": st["filter_hits"],
                                      "candidates": st["candidates"]}
-        if vname != "exact":
+        if vname in ("learned", "sample"):
             variants[vname]["estimator"] = {k: st[k] for k in ("est_predicted", "est_capacity", "est_overflow_tiles",
                                                                 "est_tiles", "est_mean_rel_err", "est_max_rel_err")}
     out = {

@@ -192,6 +192,7 @@ static JoinArgs make_args(gdsj_handle *h) {
     a.ntiles = h->G.ntiles;
     a.cellStart = h->G.cellStart.as<uint32_t>();
     a.cellCoord = h->G.cellCoord.as<uint32_t>();
+    a.cellKey = h->G.cellKey.as<uint64_t>();
     a.hkeys = h->G.hkeys.as<uint64_t>();
     a.hvals = h->G.hvals.as<uint32_t>();
     a.ga = h->G.ga;

@@ -192,6 +192,7 @@ void make_args(gdsj_batch *b, uint32_t *counts, uint64_t *offsets, uint32_t *ids
         a.ntiles = g.ntiles;
         a.cellStart = g.cellStart.as<uint32_t>();
         a.cellCoord = g.cellCoord.as<uint32_t>();
+        a.cellKey = g.cellKey.as<uint64_t>();
         a.hkeys = g.hkeys.as<uint64_t>();
         a.hvals = g.hvals.as<uint32_t>();
         a.ga = g.ga;

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
         uint2 run = make_uint2(0u, 0u);
         if (lane < nrw)
             run = pre ? a.runs[(uint64_t)cell * a.runStride + cw + lane]
-                      : combo_run(cw + lane, cc, glo, ghi, ga, a.cellStart, a.hkeys, a.hvals);
+                      : combo_run(cw + lane, cc, glo, ghi, ga, a.cellStart, a.cellKey, a.hkeys, a.hvals);
         const uint32_t len = run.y - run.x;
         uint32_t incl = len;
 #pragma unroll

@@ -10,8 +10,10 @@
 //   cellKey   [nc]    uint64 sorted unique linear ids (gridCellLookup)
 //   cellStart [nc+1]  uint32 sorted-position range of each cell (grid)
 //   cellCoord [k][nc] uint32 cell coordinates per indexed dim
-//   hkeys/hvals [cap] open-addressing table: linear id + 1 -> cell index
-//                     (replaces the binary searches of kernel.cu:395-398)
+//   hkeys/hvals [cap] open-addressing table: row (linear id without dim 0)
+//                     + 1 -> first cell of the row, or a direct table linear
+//                     id -> cell (replaces the binary searches of
+//                     kernel.cu:395-398)
 //   tiles     [nt]    uint4  {cell, q0, q1, 0}: up to 64 queries of one cell
 #pragma once
 #include "common.h"
@@ -64,9 +66,12 @@ __device__ __forceinline__ uint32_t cell_hash(uint64_t key, uint32_t bits) {
     return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - bits));
 }
 
-// cell index of linear id lin, or -1.  Direct table when hbits == 0
-// (hvals indexed by linear id, 0xFFFFFFFF = empty); otherwise linear
-// probing in a table at most half full, so a miss ends at an empty slot.
+// Direct table (hbits == 0): cell index of linear id lin, hvals indexed by
+// linear id, 0xFFFFFFFF = empty.  Hashed (hbits > 0): the table holds rows
+// (cells equal in dims 1..k-1; their linear ids are row + c_0, so a row's
+// cells are contiguous in cell order): key row + 1 -> first cell of the row,
+// linear probing in a table at most half full (a miss ends at an empty slot).
+// Returns the cell (direct) or the row's first cell (hashed), or -1.
 __device__ __forceinline__ int32_t cell_find(const uint64_t *__restrict__ hkeys, const uint32_t *__restrict__ hvals,
                                              uint32_t bits, uint64_t lin) {
     if (bits == 0) return (int32_t)hvals[lin];
@@ -80,6 +85,23 @@ __device__ __forceinline__ int32_t cell_find(const uint64_t *__restrict__ hkeys,
         h = (h + 1) & mask;
     }
     return -1;
+}
+
+// first index i in [lo, hi) with keys[i] >= want (hi if none); galloping
+// from lo, so a short distance costs few loads
+__host__ __device__ __forceinline__ uint32_t key_lower_bound(const uint64_t *__restrict__ keys, uint32_t lo, uint32_t hi,
+                                                    uint64_t want) {
+    uint32_t step = 1, a = lo, b = lo;
+    while (b < hi && keys[b] < want) {
+        a = b + 1;
+        b = (hi - b > step) ? b + step : hi;
+        step <<= 1;
+    }
+    while (a < b) {
+        const uint32_t m = a + (b - a) / 2;
+        if (keys[m] < want) a = m + 1; else b = m;
+    }
+    return a;
 }
 
 }  // namespace gdsj

@@ -187,15 +187,19 @@ __global__ void k_soa(const T *__restrict__ data, uint32_t n, uint32_t d, const 
 }
 
 // ------------------------------------------------- cell hash + coords --
-// Open-addressing table linear id -> cell index, the replacement for the
-// per-candidate-cell binary search of the reference (kernel.cu:388-399).
-__global__ void k_hash_build(const uint64_t *__restrict__ cellKey, uint32_t nc, uint32_t bits,
+// Open-addressing table row -> first cell of the row, the replacement for
+// the per-candidate-cell binary search of the reference (kernel.cu:388-399).
+// Rows: cells equal in dims 1..k-1 (linear id minus c_0); each row's first
+// cell is inserted under row + 1.
+__global__ void k_hash_build(const uint64_t *__restrict__ cellKey, uint32_t nc, uint32_t ncells0, uint32_t bits,
                              unsigned long long *__restrict__ hkeys, uint32_t *__restrict__ hvals) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
+    const uint64_t row = cellKey[c] - cellKey[c] % ncells0;
+    if (c > 0 && cellKey[c - 1] - cellKey[c - 1] % ncells0 == row) return;  // not the row's first cell
     const uint32_t mask = (1u << bits) - 1u;
-    const unsigned long long key = (unsigned long long)cellKey[c] + 1ull;
-    uint32_t h = cell_hash(cellKey[c], bits);
+    const unsigned long long key = (unsigned long long)row + 1ull;
+    uint32_t h = cell_hash(row, bits);
     for (uint32_t probe = 0; probe <= mask; probe++) {
         unsigned long long prev = atomicCAS(&hkeys[h], 0ull, key);
         if (prev == 0ull || prev == key) {
@@ -435,7 +439,7 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         GDSJ_TRY(G.hkeys.ensure(((size_t)1 << hb) * 8));
         GDSJ_TRY(G.hvals.ensure(((size_t)1 << hb) * 4));
         GDSJ_HIP(hipMemsetAsync(G.hkeys.p, 0, ((size_t)1 << hb) * 8, s));
-        k_hash_build<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellKey.as<uint64_t>(), nc, hb,
+        k_hash_build<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellKey.as<uint64_t>(), nc, G.ga.nCells[0], hb,
                                                         G.hkeys.as<unsigned long long>(), G.hvals.as<uint32_t>());
     }
     k_cell_coords<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellKey.as<uint64_t>(), nc, G.ga, G.cellCoord.as<uint32_t>());

@@ -25,6 +25,7 @@ struct JoinArgs {
     uint32_t ntiles;
     const uint32_t *cellStart;
     const uint32_t *cellCoord;
+    const uint64_t *cellKey;
     const uint64_t *hkeys;
     const uint32_t *hvals;
     const uint2 *runs;         // nullable: precomputed neighbor runs per cell

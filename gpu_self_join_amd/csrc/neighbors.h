@@ -40,6 +40,7 @@ __device__ __forceinline__ uint32_t cand_pos(uint32_t g, uint32_t nrw, uint32_t 
 __device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK], const double *glo,
                                            const double *ghi, const GridArgs &ga,
                                            const uint32_t *__restrict__ cellStart,
+                                           const uint64_t *__restrict__ cellKey,
                                            const uint64_t *__restrict__ hkeys,
                                            const uint32_t *__restrict__ hvals, bool stamp = false) {
     uint64_t base = 0;
@@ -64,6 +65,15 @@ __device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK
     if (x0 < cc[0] && gap + glo[0] > ga.prune_lim) x0 = cc[0];
     if (x1 > cc[0] && gap + ghi[0] > ga.prune_lim) x1 = cc[0];
     if (stamp && top < 0 && !(cc[0] & 1u)) x0 = x1 = cc[0];
+    if (ga.hbits) {
+        // one probe for the row, then the cells x0..x1 of it by key
+        const int32_t r0 = cell_find(hkeys, hvals, ga.hbits, base);
+        if (r0 < 0) return run;
+        const uint32_t f = key_lower_bound(cellKey, (uint32_t)r0, ga.ncell, base + x0);
+        const uint32_t e = key_lower_bound(cellKey, f, ga.ncell, base + x1 + 1);
+        if (f < e) run = make_uint2(cellStart[f], cellStart[e]);
+        return run;
+    }
     int32_t first = -1, last = -1;
     for (uint32_t x = x0; x <= x1; x++) {
         const int32_t ci = cell_find(hkeys, hvals, ga.hbits, base + x);

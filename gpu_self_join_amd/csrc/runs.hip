@@ -18,6 +18,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uint64_t ld, GridArgs ga,
                                                     const uint32_t *__restrict__ cellStart,
                                                     const uint32_t *__restrict__ cellCoord,
+                                                    const uint64_t *__restrict__ cellKey,
                                                     const uint64_t *__restrict__ hkeys,
                                                     const uint32_t *__restrict__ hvals, uint32_t ncombo,
                                                     uint64_t stride, int prune, int stamp,
@@ -49,7 +50,7 @@ __global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uin
     for (uint32_t cw = 0; cw < ncombo; cw += GDSJ_WAVE) {
         uint2 run = make_uint2(0u, 0u);
         if (cw + lane < ncombo)
-            run = combo_run(cw + lane, cc, glo, ghi, ga, cellStart, hkeys, hvals, stamp != 0);
+            run = combo_run(cw + lane, cc, glo, ghi, ga, cellStart, cellKey, hkeys, hvals, stamp != 0);
         const bool ne = run.y > run.x;
         const uint64_t m = ballot(ne);
         if (ne) mine[cursor + mbcnt(m)] = run;
@@ -86,13 +87,13 @@ int build_runs(Grid &G, hipStream_t s, bool prune, bool stamp, bool *built) {
     const uint32_t blocks = (G.ncell + 3) / 4;
     if (G.dtype == GDSJ_DTYPE_F32)
         k_cell_runs<float><<<blocks, 256, 0, s>>>(G.xs.as<float>(), G.ld, G.ga, G.cellStart.as<uint32_t>(),
-                                                  G.cellCoord.as<uint32_t>(), G.hkeys.as<uint64_t>(),
+                                                  G.cellCoord.as<uint32_t>(), G.cellKey.as<uint64_t>(), G.hkeys.as<uint64_t>(),
                                                   G.hvals.as<uint32_t>(), (uint32_t)ncombo, ncombo, prune ? 1 : 0,
                                                   stamp ? 1 : 0, G.runs.as<uint2>(), G.nruns.as<uint32_t>(),
                                                   G.cellCand.as<uint32_t>(), G.cellHist.as<float>());
     else
         k_cell_runs<double><<<blocks, 256, 0, s>>>(G.xs.as<double>(), G.ld, G.ga, G.cellStart.as<uint32_t>(),
-                                                   G.cellCoord.as<uint32_t>(), G.hkeys.as<uint64_t>(),
+                                                   G.cellCoord.as<uint32_t>(), G.cellKey.as<uint64_t>(), G.hkeys.as<uint64_t>(),
                                                    G.hvals.as<uint32_t>(), (uint32_t)ncombo, ncombo,
                                                    prune ? 1 : 0, stamp ? 1 : 0, G.runs.as<uint2>(),
                                                    G.nruns.as<uint32_t>(),
