@@ -22,6 +22,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 / fp16 MFMA
+VALU_PEAK_TFLOPS = {4: 157.3, 8: 78.6}  # MI355X FP32 / FP64 (by DTYPE size)
 
 
 def cpu_baseline(x, eps, params, sample_frac, seed=123):
@@ -333,6 +334,14 @@ def main():
     roof = {"bound": "hbm", "kernel": kname[dom], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "alg_bytes_per_launch": kb[dom], "avg_launch_ms": round(kms[dom], 4)}
+    if s["filter_rechecks"] == 0 and dom != "compact":
+        # the plain pass evaluates every candidate pair: a subtract and an FMA
+        # per dim (3 D flops), against the FP32 / FP64 vector peak
+        vflops = 3.0 * d * s["candidates"]
+        vpeak = VALU_PEAK_TFLOPS[esz]
+        vt = vflops / (kms[dom] * 1e-3) / 1e12
+        roof["valu"] = {"achieved": round(vt, 3), "peak": vpeak, "unit": "TFLOP/s", "frac": round(vt / vpeak, 4),
+                        "alg_flops_per_launch": vflops, "candidate_pairs": s["candidates"]}
     if s["filter_rechecks"] > 0 and dom != "compact":
         # the MFMA Gram filter decides the pass: algorithmic work 2*D flops per
         # candidate pair (C = queries x candidates), against the dense
@@ -372,6 +381,11 @@ def main():
         "variants": variants,
         "grid": {"nonempty_cells": s["n_nonempty_cells"], "tiles": s["n_tiles"], "candidates": s["candidates"]},
         "speedup_vs_plain": (results["plain"]["ms_per_step"] / r["ms_per_step"]) if "plain" in results else None,
+        # the join passes alone (grid resident): counting or slotted pass, scan,
+        # compaction and fill (the grid build is in ms_per_step, not here)
+        "t_join_ms": round(sum(phase[k] for k in ("ms_estimate", "ms_count", "ms_scan", "ms_compact", "ms_fill")), 4),
+        "pairs_per_s_join": total / (sum(phase[k] for k in ("ms_estimate", "ms_count", "ms_scan", "ms_compact",
+                                                            "ms_fill")) * 1e-3),
         "filter_error": filter_error(pts, eps, base) if d >= 16 else None,
         "roofline": roof,
     }

@@ -46,7 +46,7 @@ struct gdsj_handle {
     Grid G;
     DevBuf counts, offsets, ids, data, scanTmp, status, work;
     // slotted single pass (estimator-sized per-tile result slots)
-    DevBuf slotCap, slotOff, slotUsed, slotOvf, slotIds, slotQ, scalars, estPred, estScratch;
+    DevBuf slotCap, slotOff, slotUsed, slotOvf, slotIds, slotQ, scalars, estPred, estScratch, subTiles;
     Estimator est;
     GramData gram;
     DevBuf gramStats;
@@ -151,7 +151,7 @@ int gdsj_destroy(gdsj_handle *h) {
     h->rcur.release();
     DevBuf *b[] = {&h->counts,  &h->offsets,  &h->ids,     &h->data,    &h->scanTmp, &h->status,
                    &h->work,    &h->slotCap,  &h->slotOff, &h->slotUsed, &h->slotOvf, &h->slotIds,
-                   &h->slotQ,   &h->scalars,  &h->estPred, &h->estScratch};
+                   &h->slotQ,   &h->scalars,  &h->estPred, &h->estScratch, &h->subTiles};
     h->est.release();
     h->gram.release();
     h->gramStats.release();
@@ -256,8 +256,18 @@ static int do_fill(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
         GDSJ_TRY(compact_slots(a, s));
         GDSJ_HIP(hipEventRecord(h->ev[7], s));
         if (h->novf) {
-            a.tileList = h->slotOvf.as<uint32_t>();
-            a.ntiles = h->novf;
+            // the overflowed tiles are the heaviest and few: their exact fill
+            // runs on sub-tiles (32 queries for the Gram kernel, 16 for the
+            // plain one) so that it is not bound by one wave's latency
+            const uint32_t subq = h->use_gram ? 32u : 16u;
+            const uint32_t per = (h->G.tileq + subq - 1) / subq;
+            GDSJ_TRY(h->subTiles.ensure((size_t)h->novf * per * 16 + 64));
+            uint32_t nsub = 0;
+            GDSJ_TRY(split_tiles(h->slotOvf.as<uint32_t>(), h->novf, h->G.tiles.as<uint4>(), h->G.tileq, subq,
+                                 h->subTiles.as<uint4>(), &nsub, s));
+            a.tiles = h->subTiles.as<uint4>();
+            a.tileList = nullptr;
+            a.ntiles = nsub;
             GDSJ_TRY(run_pass(h, a, PASS_FILL));
         }
     } else {

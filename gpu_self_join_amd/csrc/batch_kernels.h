@@ -57,7 +57,8 @@ __device__ __forceinline__ uint32_t batch_items(const BatchLaunch &L) {
     return L.nitemsDev ? uniform(*L.nitemsDev) : L.nitems;
 }
 
-// Candidate chunks held in VGPRs: BUDGET coordinate words per lane.  Inlined
+// Candidate chunks held in VGPRs: BUDGET coordinate words per lane; query
+// coordinates are read from memory (no register copy of the tile's queries).  Inlined
 // side by side, the dims' bodies share one register allocation, larger than
 // any of them needs alone: budget 24 keeps the kernel near 110 VGPRs (4 waves
 // per SIMD), 48 is the single-join default (more reuse, 3 waves per SIMD).
@@ -79,16 +80,16 @@ __global__ __launch_bounds__(256) void k_batch_join(const JoinArgs *__restrict__
         if (SPEC) {
             switch (a.d) {
                 case 2:
-                    join_tile<T, 2, MODE, ILPV, PASS, BatchCMax<T, 2, BUDGET>::value>(xs, a.ids, a, tile);
+                    join_tile<T, 2, MODE, ILPV, PASS, BatchCMax<T, 2, BUDGET>::value, false, 0>(xs, a.ids, a, tile);
                     continue;
                 case 4:
-                    join_tile<T, 4, MODE, ILPV, PASS, BatchCMax<T, 4, BUDGET>::value>(xs, a.ids, a, tile);
+                    join_tile<T, 4, MODE, ILPV, PASS, BatchCMax<T, 4, BUDGET>::value, false, 0>(xs, a.ids, a, tile);
                     continue;
                 case 8:
-                    join_tile<T, 8, MODE, ILPV, PASS, BatchCMax<T, 8, BUDGET>::value>(xs, a.ids, a, tile);
+                    join_tile<T, 8, MODE, ILPV, PASS, BatchCMax<T, 8, BUDGET>::value, false, 0>(xs, a.ids, a, tile);
                     continue;
                 case 16:
-                    join_tile<T, 16, MODE, ILPV, PASS, BatchCMax<T, 16, BUDGET>::value>(xs, a.ids, a, tile);
+                    join_tile<T, 16, MODE, ILPV, PASS, BatchCMax<T, 16, BUDGET>::value, false, 0>(xs, a.ids, a, tile);
                     continue;
                 default:
                     break;

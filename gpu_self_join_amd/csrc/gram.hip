@@ -40,37 +40,41 @@ uint32_t gram_tile_queries(uint32_t d, int lowp) {
 }
 
 // ------------------------------------------------------------- preparation --
-template <typename T>
-__global__ __launch_bounds__(256) void k_dim_mean(const T *__restrict__ xs, uint64_t ld, uint32_t n,
-                                                   double *__restrict__ mean) {
-    const uint32_t l = blockIdx.x;
-    double s = 0.0;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s += (double)xs[(uint64_t)l * ld + i];
-    __shared__ double red[256];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (uint32_t o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) mean[l] = n ? red[0] / n : 0.0;
-}
+// Per-dim mean and max |x - mean|: GS blocks per dim write partials, one
+// block per dim reduces them in a fixed order (deterministic).
+constexpr uint32_t GS = 64;
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_dim_maxdev(const T *__restrict__ xs, uint64_t ld, uint32_t n,
-                                                     const double *__restrict__ mean, double *__restrict__ maxdev) {
-    const uint32_t l = blockIdx.x;
-    const double o = mean[l];
-    double m = 0.0;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs((double)xs[(uint64_t)l * ld + i] - o));
+__global__ __launch_bounds__(256) void k_dim_partial(const T *__restrict__ xs, uint64_t ld, uint32_t n,
+                                                      const double *__restrict__ mean, double *__restrict__ part) {
+    const uint32_t l = blockIdx.y, g = blockIdx.x;
+    const uint32_t per = (n + GS - 1) / GS;
+    const uint32_t i0 = g * per, i1 = i0 + per < n ? i0 + per : n;
+    const bool dev = mean != nullptr;  // second phase: max deviation from the mean
+    const double o = dev ? mean[l] : 0.0;
+    double v = 0.0;
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const double x = (double)xs[(uint64_t)l * ld + i];
+        v = dev ? fmax(v, fabs(x - o)) : v + x;
+    }
     __shared__ double red[256];
-    red[threadIdx.x] = m;
+    red[threadIdx.x] = v;
     __syncthreads();
     for (uint32_t q = 128; q > 0; q >>= 1) {
-        if (threadIdx.x < q) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + q]);
+        if (threadIdx.x < q) red[threadIdx.x] = dev ? fmax(red[threadIdx.x], red[threadIdx.x + q])
+                                                    : red[threadIdx.x] + red[threadIdx.x + q];
         __syncthreads();
     }
-    if (threadIdx.x == 0) maxdev[l] = red[0];
+    if (threadIdx.x == 0) part[(uint64_t)l * GS + g] = red[0];
+}
+
+__global__ void k_dim_finish(const double *__restrict__ part, uint32_t d, uint32_t n, int dev,
+                             double *__restrict__ out) {
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= d) return;
+    double v = 0.0;
+    for (uint32_t g = 0; g < GS; g++) v = dev ? fmax(v, part[(uint64_t)l * GS + g]) : v + part[(uint64_t)l * GS + g];
+    out[l] = dev ? v : (n ? v / n : 0.0);
 }
 
 // h = lp(sigma (x - o)), l = lp(sigma (x - o) - h) in the split form
@@ -144,12 +148,15 @@ static int prepare_t(GramData &gd, const Grid &G, hipStream_t s) {
     if (gd.split) GDSJ_TRY(gd.l.ensure((size_t)n * dpad * 2 + 256));
     GDSJ_TRY(gd.w.ensure((size_t)n * 4 + 256));
     GDSJ_TRY(gd.n2.ensure((size_t)n * 8 + 256));
-    GDSJ_TRY(gd.scratch.ensure((size_t)d * 16 + 256));
+    GDSJ_TRY(gd.scratch.ensure((size_t)d * 16 + 256 + (size_t)d * GS * 8));
     double *mean = gd.scratch.as<double>();
     double *maxdev = mean + d;
+    double *part = mean + 2 * (size_t)d + 8;  // after mean, maxdev and sigma
     const T *xs = G.xs.as<T>();
-    k_dim_mean<T><<<d, 256, 0, s>>>(xs, G.ld, n, mean);
-    k_dim_maxdev<T><<<d, 256, 0, s>>>(xs, G.ld, n, mean, maxdev);
+    k_dim_partial<T><<<dim3(GS, d), 256, 0, s>>>(xs, G.ld, n, nullptr, part);
+    k_dim_finish<<<ceil_div(d, 64), 64, 0, s>>>(part, d, n, 0, mean);
+    k_dim_partial<T><<<dim3(GS, d), 256, 0, s>>>(xs, G.ld, n, mean, part);
+    k_dim_finish<<<ceil_div(d, 64), 64, 0, s>>>(part, d, n, 1, maxdev);
     // eps' = eps (1 + 2 (D+5) u_T): beyond it the DTYPE sum cannot pass the predicate
     const double uT = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
     const double eps_p = G.eps * (1.0 + 2.0 * (d + 5) * uT);

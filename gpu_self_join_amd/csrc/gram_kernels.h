@@ -76,6 +76,7 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     const T thr = (T)a.thr;
     const uint4 tl = a.tiles[tile];
     const uint32_t cell = uniform(tl.x), q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
+    if (nq == 0) return;  // an empty sub-tile (split overflowed tiles)
     const uint32_t nrb = (nq + 31) / 32;
     const GridArgs &ga = a.ga;
     const bool pre = a.runs != nullptr;

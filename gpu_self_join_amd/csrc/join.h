@@ -74,6 +74,11 @@ int join_pass(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t 
 // stamp: only the neighbor cells each cell owns (unicomp).
 int build_runs(Grid &G, hipStream_t s, bool prune, bool stamp, bool *built);
 
+// out[0 .. *nout) = the tiles of list cut into sub-tiles of <= subq queries
+// (tileq: the tiles' maximum query count; out holds nlist * ceil(tileq/subq)).
+int split_tiles(const uint32_t *list, uint32_t nlist, const uint4 *tiles, uint32_t tileq, uint32_t subq,
+                uint4 *out, uint32_t *nout, hipStream_t s);
+
 // Moves every non-overflowed tile's slot entries into CSR rows.
 int compact_slots(const JoinArgs &a, hipStream_t s);
 

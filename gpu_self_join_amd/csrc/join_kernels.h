@@ -26,6 +26,11 @@ struct CMaxFor {
     static constexpr int value = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
 };
 
+// lanes 0 .. n-1 (all lanes for n >= 64)
+__device__ __forceinline__ uint64_t prefix_mask(uint32_t n) {
+    return n >= (uint32_t)GDSJ_WAVE ? ~0ull : ((1ull << n) - 1ull);
+}
+
 // value of lane i (wave-uniform i) as a scalar
 __device__ __forceinline__ float lane_value(float v, uint32_t i) {
     return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), i));
@@ -151,7 +156,8 @@ __device__ __forceinline__ T dist_mem(const T *__restrict__ xs, uint64_t ld, uin
 // queries of each candidate in a 64-bit mask and places them behind p's own
 // entries with one atomic on p's cursor (their order within a row depends
 // on the order the tiles run).
-template <typename T, int D, int MODE, int ILPV, int PASS, int CM = 0, bool STAMP = false>
+// QRW: most query coordinate words held in VGPRs (0: queries read from memory).
+template <typename T, int D, int MODE, int ILPV, int PASS, int CM = 0, bool STAMP = false, int QRW = 16>
 __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
                                           const JoinArgs &a, const uint32_t tile) {
     constexpr int CMAX = CM > 0 ? CM : CMaxFor<T, D>::value;
@@ -162,6 +168,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
     const T thr = (T)a.thr;
     const uint4 tl = a.tiles[tile];
     const uint32_t cell = uniform(tl.x), q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
+    if (nq == 0) return;  // an empty sub-tile (split overflowed tiles)
     const GridArgs &ga = a.ga;
     const bool pre = a.runs != nullptr;
     uint32_t cc[MAXK];
@@ -195,7 +202,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
     }
     // low dims: lane l holds query l's coordinates; the query loop reads them
     // with readlane (no memory access per query)
-    constexpr bool QREG = D > 0 && D * (int)(sizeof(T) / 4) <= 16;
+    constexpr bool QREG = D > 0 && D * (int)(sizeof(T) / 4) <= QRW;
     T qv[QREG ? D : 1];
     if (QREG) {
 #pragma unroll
@@ -242,7 +249,6 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
             uint32_t cpos[CMAX];
             uint32_t cid[CMAX];
             bool valid[CMAX];
-            uint64_t vmask[CMAX];
             bool other[CMAX];                                    // STAMP: candidate outside the own cell
             uint32_t jc[STAMP && PASS == PASS_COUNT ? CMAX : 1];  // STAMP: hits of this candidate
             uint64_t jm[STAMP && PASS == PASS_FILL ? CMAX : 1];   // STAMP: queries hitting it
@@ -250,7 +256,6 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
             for (int ch = 0; ch < CMAX; ch++) {
                 const uint32_t g = cb + ch * GDSJ_WAVE + lane;
                 valid[ch] = g < total;
-                vmask[ch] = ballot(valid[ch]);
                 if (STAMP && PASS == PASS_COUNT) jc[ch] = 0;
                 if (STAMP && PASS == PASS_FILL) jm[ch] = 0;
                 uint32_t sp = cand_pos(valid[ch] ? g : 0u, nrw, rstart, rexcl);
@@ -288,7 +293,9 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
 #pragma unroll
                 for (int ch = 0; ch < CMAX; ch++) {
                     if (ch < (int)nch) {
-                        // the compare's own lane mask, restricted to valid lanes
+                        // the compare's own lane mask, restricted to the valid lanes
+                        // (a prefix of the wave: computed on the scalar unit)
+                        const uint64_t vm = prefix_mask(total - cb - (uint32_t)ch * GDSJ_WAVE);
                         uint64_t m;
                         bool hit;
                         if (D > 0) {
@@ -296,11 +303,11 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                             T acc = dist_reg<T, (D > 0 ? D : 1), MODE, (ILPV > 0 ? ILPV : 1)>(
                                 q, c[ch], thr, valid[ch], a.sc != 0, alive);
                             hit = alive && valid[ch] && acc <= thr;
-                            m = alive ? (ballot(acc <= thr) & vmask[ch]) : 0ull;
+                            m = alive ? (ballot(acc <= thr) & vm) : 0ull;
                         } else {
                             T acc = dist_mem<T>(xs, ld, qpos, cpos[ch], a.d, MODE, a.ilp);
                             hit = valid[ch] && acc <= thr;
-                            m = ballot(acc <= thr) & vmask[ch];
+                            m = ballot(acc <= thr) & vm;
                         }
                         const uint32_t pm = popc64(m);
                         if (STAMP && PASS == PASS_COUNT && hit) jc[ch]++;

@@ -109,6 +109,32 @@ __global__ __launch_bounds__(256) void k_compact(const JoinArgs a) {
     compact_tile(a, tile);
 }
 
+// Sub-tiles of at most subq queries of the tiles in list (per = tileq / subq
+// sub-tiles each; the unused ones are empty).  Rows do not depend on the
+// tiling, so the fill pass over the sub-tiles writes the same rows with more
+// waves in flight.
+__global__ void k_split_tiles(const uint32_t *__restrict__ list, uint32_t nlist, const uint4 *__restrict__ tiles,
+                              uint32_t subq, uint32_t per, uint4 *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlist * per) return;
+    const uint4 tl = tiles[list[i / per]];
+    const uint32_t j = i % per;
+    uint32_t a = tl.y + j * subq;
+    if (a > tl.z) a = tl.z;
+    const uint32_t b = a + subq < tl.z ? a + subq : tl.z;
+    out[i] = make_uint4(tl.x, a, b, 0u);
+}
+
+int split_tiles(const uint32_t *list, uint32_t nlist, const uint4 *tiles, uint32_t tileq, uint32_t subq,
+                uint4 *out, uint32_t *nout, hipStream_t s) {
+    const uint32_t per = (tileq + subq - 1) / subq;
+    *nout = nlist * per;
+    if (*nout == 0) return GDSJ_OK;
+    k_split_tiles<<<ceil_div(*nout, 256), 256, 0, s>>>(list, nlist, tiles, subq, per, out);
+    GDSJ_HIP(hipGetLastError());
+    return GDSJ_OK;
+}
+
 int compact_slots(const JoinArgs &a, hipStream_t s) {
     const uint32_t blocks = (a.ntiles + 3) / 4;
     if (blocks == 0) return GDSJ_OK;
