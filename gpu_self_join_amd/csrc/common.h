@@ -47,6 +47,23 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
 
 __device__ __forceinline__ uint32_t popc64(uint64_t m) { return (uint32_t)__popcll(m); }
 
+// lanes 0 .. n-1 (all lanes for n >= 64)
+__device__ __forceinline__ uint64_t prefix_mask(uint32_t n) {
+    return n >= (uint32_t)GDSJ_WAVE ? ~0ull : ((1ull << n) - 1ull);
+}
+
+// value of lane i (wave-uniform i) as a scalar
+__device__ __forceinline__ float lane_value(float v, uint32_t i) {
+    return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), i));
+}
+__device__ __forceinline__ double lane_value(double v, uint32_t i) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, i);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), i);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+
 template <typename U>
 __device__ __forceinline__ U uniform(U v) {
     return __builtin_amdgcn_readfirstlane(v);

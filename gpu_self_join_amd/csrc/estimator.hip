@@ -386,6 +386,26 @@ int estimator_train_rows(Estimator &E, const float *feat, const float *target, u
     return GDSJ_OK;
 }
 
+// Sum (or max) over a block of up to 1024 threads; the result in thread 0.
+// One atomic per block instead of one per wave on the shared totals.
+template <typename V, bool MAX>
+__device__ __forceinline__ V block_reduce(V v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const V o = __shfl_xor(v, off, GDSJ_WAVE);
+        v = MAX ? (o > v ? o : v) : v + o;
+    }
+    __shared__ V part[1024 / GDSJ_WAVE];
+    const uint32_t w = threadIdx.x / GDSJ_WAVE, nw = (blockDim.x + GDSJ_WAVE - 1) / GDSJ_WAVE;
+    if (lane_id() == 0) part[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (uint32_t i = 1; i < nw; i++) v = MAX ? (part[i] > v ? part[i] : v) : v + part[i];
+    __syncthreads();
+    return v;
+}
+
+constexpr uint32_t RED_BLOCK = 1024;
+
 // ------------------------------------------------------------- capacities --
 __global__ void k_capacity(const float *__restrict__ predLog2, const uint4 *__restrict__ tiles, uint32_t nt,
                            const uint32_t *__restrict__ cellCand, float alpha, float beta, double perQuery,
@@ -404,8 +424,8 @@ __global__ void k_capacity(const float *__restrict__ predLog2, const uint4 *__re
         mine = (uint64_t)pred;
         if (predOut) predOut[t] = (float)log2(1.0 + pred);
     }
-    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off, GDSJ_WAVE);
-    if (lane_id() == 0 && mine) atomicAdd(predSum, (unsigned long long)mine);
+    mine = block_reduce<uint64_t, false>(mine);
+    if (threadIdx.x == 0 && mine) atomicAdd(predSum, (unsigned long long)mine);
 }
 
 int estimator_capacity_learned(Estimator &E, const Grid &G, uint32_t *capOut, uint64_t *predicted,
@@ -418,7 +438,7 @@ int estimator_capacity_learned(Estimator &E, const Grid &G, uint32_t *capOut, ui
     GDSJ_TRY(mlp_forward(E.params.as<float>(), E.feat.as<float>(), nt, E.pred.as<float>(), s));
     GDSJ_HIP(hipMemsetAsync(E.work.p, 0, 8, s));
     if (nt)
-        k_capacity<<<ceil_div(nt, 256), 256, 0, s>>>(E.pred.as<float>(), G.tiles.as<uint4>(), nt,
+        k_capacity<<<ceil_div(nt, RED_BLOCK), RED_BLOCK, 0, s>>>(E.pred.as<float>(), G.tiles.as<uint4>(), nt,
                                                      G.cellCand.as<uint32_t>(), (float)E.alpha, 8.0f, 0.0, capOut,
                                                      E.work.as<unsigned long long>(), nullptr);
     GDSJ_HIP(hipGetLastError());
@@ -485,7 +505,7 @@ int estimator_capacity_sampling(const Grid &G, const void *joinArgs, int mode, u
     const double alpha = 0.10;  // PYTHON overestimation factor, GPU.cu:224-226
     GDSJ_HIP(hipMemsetAsync(sum, 0, 8, s));
     if (nt)
-        k_capacity<<<ceil_div(nt, 256), 256, 0, s>>>(nullptr, G.tiles.as<uint4>(), nt, G.cellCand.as<uint32_t>(),
+        k_capacity<<<ceil_div(nt, RED_BLOCK), RED_BLOCK, 0, s>>>(nullptr, G.tiles.as<uint4>(), nt, G.cellCand.as<uint32_t>(),
                                                      (float)alpha, 8.0f, n ? (double)est / n : 0.0, capOut, sum,
                                                      predLog2Out);
     GDSJ_HIP(hipGetLastError());
@@ -505,12 +525,9 @@ __global__ void k_est_error(const float *__restrict__ predLog2, const uint4 *__r
         const double p = fmax(exp2((double)predLog2[t]) - 1.0, 0.0);
         rel = fabs(p - (double)y) / fmax((double)y, 1.0);
     }
-    double sm = rel, mx = rel;
-    for (int off = 32; off > 0; off >>= 1) {
-        sm += __shfl_xor(sm, off, GDSJ_WAVE);
-        mx = fmax(mx, __shfl_xor(mx, off, GDSJ_WAVE));
-    }
-    if (lane_id() == 0) {
+    const double sm = block_reduce<double, false>(rel);
+    const double mx = block_reduce<double, true>(rel);
+    if (threadIdx.x == 0) {
         atomicAdd(sumRel, sm);
         atomicMax(maxRelBits, (unsigned long long)__double_as_longlong(mx));
     }
@@ -525,7 +542,7 @@ int estimator_error(const Grid &G, const float *predLog2, const uint32_t *counts
     GDSJ_HIP(hipMemsetAsync(scratch.p, 0, 16, s));
     double *sumRel = scratch.as<double>();
     unsigned long long *mx = (unsigned long long *)(sumRel + 1);
-    k_est_error<<<ceil_div(nt, 256), 256, 0, s>>>(predLog2, G.tiles.as<uint4>(), nt, G.ids.as<uint32_t>(), counts,
+    k_est_error<<<ceil_div(nt, RED_BLOCK), RED_BLOCK, 0, s>>>(predLog2, G.tiles.as<uint4>(), nt, G.ids.as<uint32_t>(), counts,
                                                   sumRel, mx);
     double h[2] = {0, 0};
     GDSJ_HIP(hipMemcpyAsync(h, scratch.p, 16, hipMemcpyDeviceToHost, s));

@@ -41,45 +41,62 @@ __global__ void k_sample_gather(const T *__restrict__ data, uint32_t n, uint32_t
     out[(uint64_t)l * m + j] = data[(uint64_t)j * 100u * d + l];
 }
 
-// one thread per dimension: the reference's sequential sums, unfused
+// One wave per dimension: the reference's sequential sums, unfused, in
+// sample order.  The wave stages VB x 64 samples in registers (the next
+// stage's loads in flight while the current one is summed) and the dependent
+// add chain reads them with readlane, so it never waits on memory.
 template <typename T>
-__global__ void k_variance(const T *__restrict__ samples, uint32_t m, uint32_t d, T denom,
-                           T *__restrict__ var) {
-    uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= d) return;
-    const T *col = samples + (uint64_t)l * m;
-    // loads are batched ahead of the dependent add chain (same order of adds)
-    constexpr uint32_t B = 32;
-    T sum = 0;
-    uint32_t j = 0;
-    for (; j + B <= m; j += B) {
-        T v[B];
+__device__ __forceinline__ T stage_sum(const T (&v)[4], uint32_t cnt, T sum, T avg, bool sq) {
 #pragma unroll
-        for (uint32_t u = 0; u < B; u++) v[u] = col[j + u];
-#pragma unroll
-        for (uint32_t u = 0; u < B; u++) sum = sum + v[u];
-    }
-    for (; j < m; j++) sum = sum + col[j];
-    const T avg = sum / denom;
-    sum = 0;
-    j = 0;
-    for (; j + B <= m; j += B) {
-        T v[B];
-#pragma unroll
-        for (uint32_t u = 0; u < B; u++) v[u] = col[j + u];
-#pragma unroll
-        for (uint32_t u = 0; u < B; u++) {
-            T t = v[u] - avg;
-            T sq = t * t;
-            sum = sum + sq;
+    for (int u = 0; u < 4; u++) {
+#pragma unroll 8
+        for (uint32_t i = 0; i < (uint32_t)GDSJ_WAVE; i++) {
+            if ((uint32_t)u * GDSJ_WAVE + i < cnt) {
+                const T x = lane_value(v[u], i);
+                if (sq) {
+                    const T t = x - avg;
+                    const T q = t * t;
+                    sum = sum + q;
+                } else {
+                    sum = sum + x;
+                }
+            }
         }
     }
-    for (; j < m; j++) {
-        T t = col[j] - avg;
-        T sq = t * t;
-        sum = sum + sq;
+    return sum;
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_variance(const T *__restrict__ samples, uint32_t m, uint32_t d, T denom,
+                                                  T *__restrict__ var) {
+    constexpr uint32_t VB = 4, STAGE = VB * GDSJ_WAVE;
+    const uint32_t l = blockIdx.x;
+    if (l >= d) return;
+    const T *col = samples + (uint64_t)l * m;
+    const uint32_t lane = lane_id();
+    T avg = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        T sum = 0;
+        T v[VB], nx[VB];
+#pragma unroll
+        for (uint32_t u = 0; u < VB; u++) {
+            const uint32_t idx = u * GDSJ_WAVE + lane;
+            v[u] = idx < m ? col[idx] : (T)0;
+        }
+        for (uint32_t j = 0; j < m; j += STAGE) {
+#pragma unroll
+            for (uint32_t u = 0; u < VB; u++) {
+                const uint32_t idx = j + STAGE + u * GDSJ_WAVE + lane;
+                nx[u] = idx < m ? col[idx] : (T)0;
+            }
+            const uint32_t cnt = m - j < STAGE ? m - j : STAGE;
+            sum = stage_sum<T>(v, cnt, sum, avg, pass == 1);
+#pragma unroll
+            for (uint32_t u = 0; u < VB; u++) v[u] = nx[u];
+        }
+        if (pass == 0) avg = sum / denom;
+        else if (lane == 0) var[l] = sum / denom;
     }
-    var[l] = sum / denom;
 }
 
 // ------------------------------------------------------------ extent --
@@ -249,8 +266,13 @@ __global__ void k_tiles_fill(const uint32_t *__restrict__ cellStart, uint32_t nc
         tiles[t] = make_uint4(c, q, q1, 0u);
         if (order) {
             // QUERYREORDER (GPU.cu:1491-1525): queries of the most occupied
-            // cells first; stable, so ties keep cell order
-            tileKey[t] = (uint64_t)(~(s1 - s0));
+            // cells first; stable, so ties keep cell order.  order 2: by
+            // occupancy class (power of two) only, so each class is one
+            // spatially ordered run of tiles (neighboring tiles share
+            // candidates in L2)
+            const uint32_t occ = s1 - s0;
+            const uint32_t cls = 31u - (uint32_t)__builtin_clz(occ | 1u);  // floor(log2(occ))
+            tileKey[t] = order == 2 ? (uint64_t)(31u - cls) : (uint64_t)(~occ);
             tileVal[t] = t;
         }
     }
@@ -319,7 +341,7 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         k_sample_gather<T><<<(uint32_t)((tot + 255) / 256), 256, 0, s>>>(data, n, d, m, samp);
         const T inv_sample = (T)(1.0 / (100 * 1.0));
         const T denom = (T)n * inv_sample;
-        k_variance<T><<<(d + 63) / 64, 64, 0, s>>>(samp, m, d, denom, var);
+        k_variance<T><<<d, 64, 0, s>>>(samp, m, d, denom, var);
         std::vector<T> hv(d);
         GDSJ_HIP(hipMemcpyAsync(hv.data(), var, d * sizeof(T), hipMemcpyDeviceToHost, s));
         GDSJ_HIP(hipStreamSynchronize(s));
@@ -469,10 +491,10 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         uint32_t *tv = G.tileVal.as<uint32_t>();
         uint4 *tilesB = tilesA + nt;
         k_tiles_fill<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellStart.as<uint32_t>(), nc, G.tileq, tileOff, tilesB,
-                                                        tk, tv, 1);
+                                                        tk, tv, order_tiles == 2 ? 2 : 1);
         GDSJ_TRY(G.tmp.ensure(radix_tmp_bytes(nt) + 1024));
         int alt2 = 0;
-        GDSJ_TRY(radix_sort_pairs(tk, tv, tk + nt, tv + nt, nt, 32, G.tmp.p, s, &alt2));
+        GDSJ_TRY(radix_sort_pairs(tk, tv, tk + nt, tv + nt, nt, order_tiles == 2 ? 8 : 32, G.tmp.p, s, &alt2));
         k_tiles_permute<<<ceil_div(nt, 256), 256, 0, s>>>(tilesB, alt2 ? tv + nt : tv, nt, tilesA);
     } else {
         k_tiles_fill<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellStart.as<uint32_t>(), nc, G.tileq, tileOff, tilesA,

@@ -14,6 +14,8 @@
 // predicate sqrt(acc) <= eps is evaluated as acc <= thr with thr the
 // largest DTYPE value whose correctly rounded square root is <= eps.
 #pragma once
+#include <type_traits>
+
 #include "join.h"
 #include "neighbors.h"
 
@@ -25,22 +27,6 @@ struct CMaxFor {
     static constexpr int raw = D > 0 ? 48 / (D * words) : 1;
     static constexpr int value = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
 };
-
-// lanes 0 .. n-1 (all lanes for n >= 64)
-__device__ __forceinline__ uint64_t prefix_mask(uint32_t n) {
-    return n >= (uint32_t)GDSJ_WAVE ? ~0ull : ((1ull << n) - 1ull);
-}
-
-// value of lane i (wave-uniform i) as a scalar
-__device__ __forceinline__ float lane_value(float v, uint32_t i) {
-    return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), i));
-}
-__device__ __forceinline__ double lane_value(double v, uint32_t i) {
-    const uint64_t b = (uint64_t)__double_as_longlong(v);
-    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, i);
-    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), i);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
 
 // Squared distance in the reference's order.  q: query coords (uniform),
 // c: candidate coords (this lane).  Returns false early (acc > thr on every
@@ -236,6 +222,18 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
         const uint32_t total = uniform(shfl_u32(incl, nrw - 1));
         if (total == 0) continue;
         work += total;
+        // fill pass: when every query's row can take all of this window's
+        // candidates below the output capacity, the writes skip the check
+        bool roomy = false;
+        if (PASS == PASS_FILL) {
+            uint64_t mx = lane < nq ? cur_l : 0ull;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint64_t o = __shfl_xor(mx, off, GDSJ_WAVE);
+                mx = o > mx ? o : mx;
+            }
+            roomy = uniform64(mx) + total <= a.cap;
+        }
         const uint32_t rexcl = incl - len;
         const uint32_t rstart = run.x;
         // a valid position for lanes past the end: first candidate of the window
@@ -268,79 +266,85 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                 }
                 if (NEED_IDS) cid[ch] = (ch < (int)nch) ? ids[sp] : 0u;
             }
-            for (uint32_t qi = 0; qi < nq; qi++) {
-                const uint32_t qpos = q0 + qi;
-                T q[D > 0 ? D : 1];
-                if (QREG) {
-#pragma unroll
-                    for (int l = 0; l < (D > 0 ? D : 1); l++) q[l] = lane_value(qv[l], qi);
-                } else if (D > 0) {
-#pragma unroll
-                    for (int l = 0; l < (D > 0 ? D : 1); l++) q[l] = xs[(uint64_t)l * ld + qpos];
-                }
-                uint32_t tot = 0;
-                uint64_t cur = 0;
-                uint32_t *orow = nullptr;  // this query's next CSR slot (wave-uniform)
-                uint32_t room = 0;         // slots left before the output capacity
-                if (PASS == PASS_FILL) {
-                    uint32_t lo = __builtin_amdgcn_readlane((uint32_t)cur_l, qi);
-                    uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(cur_l >> 32), qi);
-                    cur = ((uint64_t)hi << 32) | lo;
-                    orow = a.out + cur;
-                    const uint64_t left = cur < a.cap ? a.cap - cur : 0ull;
-                    room = left > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)left;
-                }
-#pragma unroll
-                for (int ch = 0; ch < CMAX; ch++) {
-                    if (ch < (int)nch) {
-                        // the compare's own lane mask, restricted to the valid lanes
-                        // (a prefix of the wave: computed on the scalar unit)
-                        const uint64_t vm = prefix_mask(total - cb - (uint32_t)ch * GDSJ_WAVE);
-                        uint64_t m;
-                        bool hit;
-                        if (D > 0) {
-                            bool alive;
-                            T acc = dist_reg<T, (D > 0 ? D : 1), MODE, (ILPV > 0 ? ILPV : 1)>(
-                                q, c[ch], thr, valid[ch], a.sc != 0, alive);
-                            hit = alive && valid[ch] && acc <= thr;
-                            m = alive ? (ballot(acc <= thr) & vm) : 0ull;
-                        } else {
-                            T acc = dist_mem<T>(xs, ld, qpos, cpos[ch], a.d, MODE, a.ilp);
-                            hit = valid[ch] && acc <= thr;
-                            m = ballot(acc <= thr) & vm;
-                        }
-                        const uint32_t pm = popc64(m);
-                        if (STAMP && PASS == PASS_COUNT && hit) jc[ch]++;
-                        if (STAMP && PASS == PASS_FILL && hit) jm[ch] |= 1ull << qi;
-                        if (PASS == PASS_FILL) {
-                            if (hit) {
-                                const uint32_t r = mbcnt(m);
-                                if (r < room) orow[r] = cid[ch]; else overflow = true;
+            // the query loop, unswitched on the fill pass's capacity check
+            auto queries = [&](auto roomy_c) {
+                constexpr bool RY = decltype(roomy_c)::value;
+                for (uint32_t qi = 0; qi < nq; qi++) {
+                    const uint32_t qpos = q0 + qi;
+                    T q[D > 0 ? D : 1];
+                    if (QREG) {
+    #pragma unroll
+                        for (int l = 0; l < (D > 0 ? D : 1); l++) q[l] = lane_value(qv[l], qi);
+                    } else if (D > 0) {
+    #pragma unroll
+                        for (int l = 0; l < (D > 0 ? D : 1); l++) q[l] = xs[(uint64_t)l * ld + qpos];
+                    }
+                    uint32_t tot = 0;
+                    uint64_t cur = 0;
+                    uint32_t *orow = nullptr;  // this query's next CSR slot (wave-uniform)
+                    uint32_t room = 0;         // slots left before the output capacity
+                    if (PASS == PASS_FILL) {
+                        uint32_t lo = __builtin_amdgcn_readlane((uint32_t)cur_l, qi);
+                        uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(cur_l >> 32), qi);
+                        cur = ((uint64_t)hi << 32) | lo;
+                        orow = a.out + cur;
+                        const uint64_t left = cur < a.cap ? a.cap - cur : 0ull;
+                        room = left > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)left;
+                    }
+    #pragma unroll
+                    for (int ch = 0; ch < CMAX; ch++) {
+                        if (ch < (int)nch) {
+                            // the compare's own lane mask, restricted to the valid lanes
+                            // (a prefix of the wave: computed on the scalar unit)
+                            const uint64_t vm = prefix_mask(total - cb - (uint32_t)ch * GDSJ_WAVE);
+                            uint64_t m;
+                            bool hit;
+                            if (D > 0) {
+                                bool alive;
+                                T acc = dist_reg<T, (D > 0 ? D : 1), MODE, (ILPV > 0 ? ILPV : 1)>(
+                                    q, c[ch], thr, valid[ch], a.sc != 0, alive);
+                                hit = alive && valid[ch] && acc <= thr;
+                                m = alive ? (ballot(acc <= thr) & vm) : 0ull;
+                            } else {
+                                T acc = dist_mem<T>(xs, ld, qpos, cpos[ch], a.d, MODE, a.ilp);
+                                hit = valid[ch] && acc <= thr;
+                                m = ballot(acc <= thr) & vm;
                             }
-                            orow += pm;
-                            room = room > pm ? room - pm : 0u;
-                            cur += pm;
-                        } else {
-                            if (PASS == PASS_SLOT && pm) {
+                            const uint32_t pm = popc64(m);
+                            if (STAMP && PASS == PASS_COUNT && hit) jc[ch]++;
+                            if (STAMP && PASS == PASS_FILL && hit) jm[ch] |= 1ull << qi;
+                            if (PASS == PASS_FILL) {
                                 if (hit) {
-                                    const uint32_t p = scur + mbcnt(m);
-                                    if (p < scap) {
-                                        a.slotIds[sbase + p] = cid[ch];
-                                        a.slotQ[sbase + p] = (uint8_t)qi;
-                                    }
+                                    const uint32_t r = mbcnt(m);
+                                    if (RY || r < room) orow[r] = cid[ch]; else overflow = true;
                                 }
-                                scur += pm;
+                                orow += pm;
+                                if (!RY) room = room > pm ? room - pm : 0u;
+                                cur += pm;
+                            } else {
+                                if (PASS == PASS_SLOT && pm) {
+                                    if (hit) {
+                                        const uint32_t p = scur + mbcnt(m);
+                                        if (p < scap) {
+                                            a.slotIds[sbase + p] = cid[ch];
+                                            a.slotQ[sbase + p] = (uint8_t)qi;
+                                        }
+                                    }
+                                    scur += pm;
+                                }
+                                tot += pm;
                             }
-                            tot += pm;
                         }
                     }
+                    if (PASS == PASS_FILL) {
+                        if (lane == qi) cur_l = cur;
+                    } else {
+                        if (lane == qi) cnt_l += tot;
+                    }
                 }
-                if (PASS == PASS_FILL) {
-                    if (lane == qi) cur_l = cur;
-                } else {
-                    if (lane == qi) cnt_l += tot;
-                }
-            }
+            };
+            if (PASS == PASS_FILL && roomy) queries(std::true_type{});
+            else queries(std::false_type{});
             if (STAMP) {
                 // the reverse direction of this chunk batch's hits
 #pragma unroll

@@ -60,7 +60,8 @@ typedef struct gdsj_params {
     uint32_t shortcircuit;  /* SHORTCIRCUIT (speed only) */
     uint32_t reorder;       /* REORDER: permute dims by descending variance */
     uint32_t stamp;         /* STAMP / unicomp (speed only; both directions emitted) */
-    uint32_t queryreorder;  /* QUERYREORDER: heaviest tiles first (speed only) */
+    uint32_t queryreorder;  /* QUERYREORDER: 1 heaviest tiles first; 2 by occupancy class
+                               (powers of two), cell order within a class (speed only) */
     uint32_t filter;        /* GDSJ_FILTER_* */
     uint32_t estimator;     /* GDSJ_EST_* */
     uint32_t sort_neighbors;/* 1: each point's neighbor ids ascending */
