@@ -28,6 +28,29 @@ struct CMaxFor {
     static constexpr int value = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
 };
 
+// One padded row of the row-major copy (AosRow) into c[0..D): 8-byte rows
+// with one 8-byte load, longer rows with 16-byte loads.
+template <typename T, int D>
+__device__ __forceinline__ void load_row(const T *__restrict__ row, T (&c)[D]) {
+    constexpr int PAD = AosRow<T, D>::pad;
+    constexpr int E = AosRow<T, D>::elems;
+    T v[E > 0 ? E : 1];
+    if constexpr (PAD == 0) {
+        return;
+    } else if constexpr (PAD == 8) {
+        const uint2 u = *reinterpret_cast<const uint2 *>(row);
+        __builtin_memcpy(v, &u, 8);
+    } else {
+#pragma unroll
+        for (int i = 0; i < PAD / 16; i++) {
+            const uint4 u = reinterpret_cast<const uint4 *>(row)[i];
+            __builtin_memcpy(reinterpret_cast<char *>(v) + 16 * i, &u, 16);
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < D; l++) c[l] = v[l];
+}
+
 // Squared distance in the reference's order.  q: query coords (uniform),
 // c: candidate coords (this lane).  Returns false early (acc > thr on every
 // lane of the wave that is still a candidate) when short-circuiting; the
@@ -189,6 +212,9 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
     // low dims: lane l holds query l's coordinates; the query loop reads them
     // with readlane (no memory access per query)
     constexpr bool QREG = D > 0 && D * (int)(sizeof(T) / 4) <= QRW;
+    // candidates from the row-major copy when it exists (uniform)
+    constexpr bool AOS = D > 0 && AosRow<T, (D > 0 ? D : 1)>::elems > 0;
+    const T *xa = (AOS && a.dpa == (uint32_t)AosRow<T, (D > 0 ? D : 1)>::elems) ? (const T *)a.xa : nullptr;
     T qv[QREG ? D : 1];
     if (QREG) {
 #pragma unroll
@@ -261,8 +287,18 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                 cpos[ch] = sp;
                 other[ch] = STAMP && valid[ch] && (sp < own0 || sp >= own1);
                 if (D > 0 && ch < (int)nch) {
+                    bool done = false;
+                    if constexpr (AOS) {
+                        if (xa) {
+                            load_row<T, (D > 0 ? D : 1)>(xa + (uint64_t)sp * AosRow<T, (D > 0 ? D : 1)>::elems,
+                                                         c[ch]);
+                            done = true;
+                        }
+                    }
+                    if (!done) {
 #pragma unroll
-                    for (int l = 0; l < (D > 0 ? D : 1); l++) c[ch][l] = xs[(uint64_t)l * ld + sp];
+                        for (int l = 0; l < (D > 0 ? D : 1); l++) c[ch][l] = xs[(uint64_t)l * ld + sp];
+                    }
                 }
                 if (NEED_IDS) cid[ch] = (ch < (int)nch) ? ids[sp] : 0u;
             }
