@@ -187,8 +187,6 @@ static JoinArgs make_args(gdsj_handle *h) {
     JoinArgs a{};
     a.xs = h->G.xs.p;
     a.ld = h->G.ld;
-    a.xa = h->G.dpa ? h->G.xa.p : nullptr;
-    a.dpa = h->G.dpa;
     a.ids = h->G.ids.as<uint32_t>();
     a.tiles = h->G.tiles.as<uint4>();
     a.ntiles = h->G.ntiles;

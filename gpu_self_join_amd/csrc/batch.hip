@@ -187,8 +187,6 @@ void make_args(gdsj_batch *b, uint32_t *counts, uint64_t *offsets, uint32_t *ids
         JoinArgs &a = b->hArgs[i];
         a.xs = g.xs.p;
         a.ld = g.ld;
-        a.xa = g.dpa ? g.xa.p : nullptr;
-        a.dpa = g.dpa;
         a.ids = g.ids.as<uint32_t>();
         a.tiles = g.tiles.as<uint4>();
         a.ntiles = g.ntiles;

@@ -6,7 +6,6 @@
 //
 // HBM layout after build (sorted = points ordered by (cell linear id, id)):
 //   xs        [D][ld] DTYPE  SoA coordinates in sorted order, dims permuted
-//   xa        [n][dpa] DTYPE the same, row-major, rows padded (dims <= 64 bytes)
 //   ids       [n]     uint32 sorted position -> original point id
 //   cellKey   [nc]    uint64 sorted unique linear ids (gridCellLookup)
 //   cellStart [nc+1]  uint32 sorted-position range of each cell (grid)
@@ -24,23 +23,6 @@ namespace gdsj {
 constexpr int MAXK = 12;
 constexpr int MAXD = 1024;
 constexpr int RUN_HIST_BINS = 6;  // neighbor-run length histogram per cell
-
-// Row-major copy of the sorted points for the join's candidate loads: a
-// candidate's coordinates are one padded row (8, 16, 32 or 64 bytes) read
-// with vector loads, so a short run of candidates touches few cache lines
-// (in 6-D a run holds a few points, and the D separate arrays of the SoA
-// layout would each be touched at every run).  Rows of dims <= 64 bytes.
-template <typename T, int D>
-struct AosRow {
-    static constexpr int bytes = D * (int)sizeof(T);
-    static constexpr int pad = bytes <= 8 ? 8 : bytes <= 16 ? 16 : bytes <= 32 ? 32 : bytes <= 64 ? 64 : 0;
-    static constexpr int elems = pad / (int)sizeof(T);  // 0: no row-major copy
-};
-inline uint32_t aos_elems(uint32_t d, uint32_t esize) {
-    const uint32_t b = d * esize;
-    const uint32_t pad = b <= 8 ? 8 : b <= 16 ? 16 : b <= 32 ? 32 : b <= 64 ? 64 : 0;
-    return pad / esize;
-}
 
 // what the join kernels need to enumerate adjacent cells
 struct GridArgs {
@@ -66,9 +48,8 @@ struct Grid {
     uint64_t totalCells = 0;
     uint32_t ncell = 0, ntiles = 0, tileq = 64;
     uint64_t ld = 0;  // leading dimension of xs
-    uint32_t dpa = 0; // elements per row of xa (0: no row-major copy)
     // device buffers
-    DevBuf xs, xa, ids, cellKey, cellStart, cellCoord, hkeys, hvals, tiles;
+    DevBuf xs, ids, cellKey, cellStart, cellCoord, hkeys, hvals, tiles;
     DevBuf runs, nruns, cellCand, cellHist;  // per-cell neighbor runs (build_runs)
     uint64_t runStride = 0;
     bool haveRuns = false;

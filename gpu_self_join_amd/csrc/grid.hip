@@ -41,44 +41,45 @@ __global__ void k_sample_gather(const T *__restrict__ data, uint32_t n, uint32_t
     out[(uint64_t)l * m + j] = data[(uint64_t)j * 100u * d + l];
 }
 
-// One thread per dimension: the reference's sequential sums, unfused, in
-// sample order.  Loads run two batches ahead of the dependent add chain
-// (double buffer), so the chain does not wait on memory.
+// one thread per dimension: the reference's sequential sums, unfused
 template <typename T>
 __global__ void k_variance(const T *__restrict__ samples, uint32_t m, uint32_t d, T denom,
                            T *__restrict__ var) {
-    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= d) return;
     const T *col = samples + (uint64_t)l * m;
+    // loads are batched ahead of the dependent add chain (same order of adds)
     constexpr uint32_t B = 32;
-    T avg = 0;
-    for (int pass = 0; pass < 2; pass++) {
-        T sum = 0;
-        T cur[B], nxt[B];
+    T sum = 0;
+    uint32_t j = 0;
+    for (; j + B <= m; j += B) {
+        T v[B];
 #pragma unroll
-        for (uint32_t u = 0; u < B; u++) cur[u] = u < m ? col[u] : (T)0;
-        for (uint32_t j = 0; j < m; j += B) {
+        for (uint32_t u = 0; u < B; u++) v[u] = col[j + u];
 #pragma unroll
-            for (uint32_t u = 0; u < B; u++) nxt[u] = j + B + u < m ? col[j + B + u] : (T)0;
-            const uint32_t cnt = m - j < B ? m - j : B;
-#pragma unroll
-            for (uint32_t u = 0; u < B; u++) {
-                if (u < cnt) {
-                    if (pass == 0) {
-                        sum = sum + cur[u];
-                    } else {
-                        const T t = cur[u] - avg;
-                        const T sq = t * t;
-                        sum = sum + sq;
-                    }
-                }
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < B; u++) cur[u] = nxt[u];
-        }
-        if (pass == 0) avg = sum / denom;
-        else var[l] = sum / denom;
+        for (uint32_t u = 0; u < B; u++) sum = sum + v[u];
     }
+    for (; j < m; j++) sum = sum + col[j];
+    const T avg = sum / denom;
+    sum = 0;
+    j = 0;
+    for (; j + B <= m; j += B) {
+        T v[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++) v[u] = col[j + u];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++) {
+            T t = v[u] - avg;
+            T sq = t * t;
+            sum = sum + sq;
+        }
+    }
+    for (; j < m; j++) {
+        T t = col[j] - avg;
+        T sq = t * t;
+        sum = sum + sq;
+    }
+    var[l] = sum / denom;
 }
 
 // ------------------------------------------------------------ extent --
@@ -169,8 +170,8 @@ __global__ void k_cells(const uint64_t *__restrict__ keys, const uint32_t *__res
 // SoA gather in sorted order with the dimension permutation applied
 template <typename T>
 __global__ void k_soa(const T *__restrict__ data, uint32_t n, uint32_t d, const uint32_t *__restrict__ perm,
-                      const uint32_t *__restrict__ order, T *__restrict__ xs, uint64_t ld, T *__restrict__ xa,
-                      uint32_t dpa, uint32_t *__restrict__ ids, uint32_t *__restrict__ nonfinite) {
+                      const uint32_t *__restrict__ order, T *__restrict__ xs, uint64_t ld,
+                      uint32_t *__restrict__ ids, uint32_t *__restrict__ nonfinite) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     uint32_t id = order[s];
@@ -181,10 +182,7 @@ __global__ void k_soa(const T *__restrict__ data, uint32_t n, uint32_t d, const 
         T v = row[perm[l]];
         bad |= !isfinite(v);
         xs[(uint64_t)l * ld + s] = v;
-        if (xa) xa[(uint64_t)s * dpa + l] = v;
     }
-    if (xa)
-        for (uint32_t l = d; l < dpa; l++) xa[(uint64_t)s * dpa + l] = (T)0;
     if (bad) atomicOr(nonfinite, 1u);
 }
 
@@ -272,7 +270,7 @@ __global__ void k_tiles_permute(const uint4 *__restrict__ in, const uint32_t *__
 
 // ---------------------------------------------------------------- host --
 void Grid::release() {
-    DevBuf *b[] = {&xs, &xa, &ids, &cellKey, &cellStart, &cellCoord, &hkeys, &hvals, &tiles, &keyA, &keyB,
+    DevBuf *b[] = {&xs, &ids, &cellKey, &cellStart, &cellCoord, &hkeys, &hvals, &tiles, &keyA, &keyB,
                    &valB, &tmp, &aux, &flag, &tileKey, &tileVal, &cellAux, &scalars, &runs, &nruns, &cellCand, &cellHist};
     for (DevBuf *p : b) p->release();
 }
@@ -409,10 +407,7 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     uint32_t *nonfinite = G.scalars.as<uint32_t>();
     uint32_t *idsOut = alt ? vA : vB;  // the value buffer the sort did not end in
     GDSJ_HIP(hipMemsetAsync(nonfinite, 0, 4, s));
-    G.dpa = aos_elems(d, (uint32_t)sizeof(T));
-    if (G.dpa) GDSJ_TRY(G.xa.ensure((size_t)n * G.dpa * sizeof(T) + 256));
-    k_soa<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, dperm, svals, G.xs.as<T>(), G.ld,
-                                               G.dpa ? G.xa.as<T>() : nullptr, G.dpa, idsOut, nonfinite);
+    k_soa<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, dperm, svals, G.xs.as<T>(), G.ld, idsOut, nonfinite);
     if (idsOut != G.ids.as<uint32_t>()) std::swap(G.ids, G.valB);
     uint32_t hdr[3] = {0, 0, 0};
     GDSJ_HIP(hipMemcpyAsync(&hdr[0], cidx + n, 4, hipMemcpyDeviceToHost, s));

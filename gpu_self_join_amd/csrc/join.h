@@ -19,8 +19,6 @@ enum JoinPass : int { PASS_COUNT = 0, PASS_FILL = 1, PASS_SLOT = 2 };
 struct JoinArgs {
     const void *xs;
     uint64_t ld;
-    const void *xa;  // row-major copy (Grid::xa) or null
-    uint32_t dpa;    // its row length in elements
     const uint32_t *ids;
     const uint4 *tiles;
     const uint32_t *tileList;  // nullable: process tiles[tileList[t]] for t < ntiles

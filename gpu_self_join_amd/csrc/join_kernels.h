@@ -14,8 +14,6 @@
 // predicate sqrt(acc) <= eps is evaluated as acc <= thr with thr the
 // largest DTYPE value whose correctly rounded square root is <= eps.
 #pragma once
-#include <type_traits>
-
 #include "join.h"
 #include "neighbors.h"
 
@@ -27,29 +25,6 @@ struct CMaxFor {
     static constexpr int raw = D > 0 ? 48 / (D * words) : 1;
     static constexpr int value = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
 };
-
-// One padded row of the row-major copy (AosRow) into c[0..D): 8-byte rows
-// with one 8-byte load, longer rows with 16-byte loads.
-template <typename T, int D>
-__device__ __forceinline__ void load_row(const T *__restrict__ row, T (&c)[D]) {
-    constexpr int PAD = AosRow<T, D>::pad;
-    constexpr int E = AosRow<T, D>::elems;
-    T v[E > 0 ? E : 1];
-    if constexpr (PAD == 0) {
-        return;
-    } else if constexpr (PAD == 8) {
-        const uint2 u = *reinterpret_cast<const uint2 *>(row);
-        __builtin_memcpy(v, &u, 8);
-    } else {
-#pragma unroll
-        for (int i = 0; i < PAD / 16; i++) {
-            const uint4 u = reinterpret_cast<const uint4 *>(row)[i];
-            __builtin_memcpy(reinterpret_cast<char *>(v) + 16 * i, &u, 16);
-        }
-    }
-#pragma unroll
-    for (int l = 0; l < D; l++) c[l] = v[l];
-}
 
 // Squared distance in the reference's order.  q: query coords (uniform),
 // c: candidate coords (this lane).  Returns false early (acc > thr on every
@@ -166,7 +141,10 @@ __device__ __forceinline__ T dist_mem(const T *__restrict__ xs, uint64_t ld, uin
 // entries with one atomic on p's cursor (their order within a row depends
 // on the order the tiles run).
 // QRW: most query coordinate words held in VGPRs (0: queries read from memory).
-template <typename T, int D, int MODE, int ILPV, int PASS, int CM = 0, bool STAMP = false, int QRW = 16>
+// UNSW: two copies of the fill pass's query loop (with and without the
+// capacity check); a single copy keeps the batch kernel's registers down.
+template <typename T, int D, int MODE, int ILPV, int PASS, int CM = 0, bool STAMP = false, int QRW = 16,
+          bool UNSW = true>
 __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
                                           const JoinArgs &a, const uint32_t tile) {
     constexpr int CMAX = CM > 0 ? CM : CMaxFor<T, D>::value;
@@ -212,9 +190,6 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
     // low dims: lane l holds query l's coordinates; the query loop reads them
     // with readlane (no memory access per query)
     constexpr bool QREG = D > 0 && D * (int)(sizeof(T) / 4) <= QRW;
-    // candidates from the row-major copy when it exists (uniform)
-    constexpr bool AOS = D > 0 && AosRow<T, (D > 0 ? D : 1)>::elems > 0;
-    const T *xa = (AOS && a.dpa == (uint32_t)AosRow<T, (D > 0 ? D : 1)>::elems) ? (const T *)a.xa : nullptr;
     T qv[QREG ? D : 1];
     if (QREG) {
 #pragma unroll
@@ -287,100 +262,22 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                 cpos[ch] = sp;
                 other[ch] = STAMP && valid[ch] && (sp < own0 || sp >= own1);
                 if (D > 0 && ch < (int)nch) {
-                    bool done = false;
-                    if constexpr (AOS) {
-                        if (xa) {
-                            load_row<T, (D > 0 ? D : 1)>(xa + (uint64_t)sp * AosRow<T, (D > 0 ? D : 1)>::elems,
-                                                         c[ch]);
-                            done = true;
-                        }
-                    }
-                    if (!done) {
 #pragma unroll
-                        for (int l = 0; l < (D > 0 ? D : 1); l++) c[ch][l] = xs[(uint64_t)l * ld + sp];
-                    }
+                    for (int l = 0; l < (D > 0 ? D : 1); l++) c[ch][l] = xs[(uint64_t)l * ld + sp];
                 }
                 if (NEED_IDS) cid[ch] = (ch < (int)nch) ? ids[sp] : 0u;
             }
-            // the query loop, unswitched on the fill pass's capacity check
-            auto queries = [&](auto roomy_c) {
-                constexpr bool RY = decltype(roomy_c)::value;
-                for (uint32_t qi = 0; qi < nq; qi++) {
-                    const uint32_t qpos = q0 + qi;
-                    T q[D > 0 ? D : 1];
-                    if (QREG) {
-    #pragma unroll
-                        for (int l = 0; l < (D > 0 ? D : 1); l++) q[l] = lane_value(qv[l], qi);
-                    } else if (D > 0) {
-    #pragma unroll
-                        for (int l = 0; l < (D > 0 ? D : 1); l++) q[l] = xs[(uint64_t)l * ld + qpos];
-                    }
-                    uint32_t tot = 0;
-                    uint64_t cur = 0;
-                    uint32_t *orow = nullptr;  // this query's next CSR slot (wave-uniform)
-                    uint32_t room = 0;         // slots left before the output capacity
-                    if (PASS == PASS_FILL) {
-                        uint32_t lo = __builtin_amdgcn_readlane((uint32_t)cur_l, qi);
-                        uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(cur_l >> 32), qi);
-                        cur = ((uint64_t)hi << 32) | lo;
-                        orow = a.out + cur;
-                        const uint64_t left = cur < a.cap ? a.cap - cur : 0ull;
-                        room = left > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)left;
-                    }
-    #pragma unroll
-                    for (int ch = 0; ch < CMAX; ch++) {
-                        if (ch < (int)nch) {
-                            // the compare's own lane mask, restricted to the valid lanes
-                            // (a prefix of the wave: computed on the scalar unit)
-                            const uint64_t vm = prefix_mask(total - cb - (uint32_t)ch * GDSJ_WAVE);
-                            uint64_t m;
-                            bool hit;
-                            if (D > 0) {
-                                bool alive;
-                                T acc = dist_reg<T, (D > 0 ? D : 1), MODE, (ILPV > 0 ? ILPV : 1)>(
-                                    q, c[ch], thr, valid[ch], a.sc != 0, alive);
-                                hit = alive && valid[ch] && acc <= thr;
-                                m = alive ? (ballot(acc <= thr) & vm) : 0ull;
-                            } else {
-                                T acc = dist_mem<T>(xs, ld, qpos, cpos[ch], a.d, MODE, a.ilp);
-                                hit = valid[ch] && acc <= thr;
-                                m = ballot(acc <= thr) & vm;
-                            }
-                            const uint32_t pm = popc64(m);
-                            if (STAMP && PASS == PASS_COUNT && hit) jc[ch]++;
-                            if (STAMP && PASS == PASS_FILL && hit) jm[ch] |= 1ull << qi;
-                            if (PASS == PASS_FILL) {
-                                if (hit) {
-                                    const uint32_t r = mbcnt(m);
-                                    if (RY || r < room) orow[r] = cid[ch]; else overflow = true;
-                                }
-                                orow += pm;
-                                if (!RY) room = room > pm ? room - pm : 0u;
-                                cur += pm;
-                            } else {
-                                if (PASS == PASS_SLOT && pm) {
-                                    if (hit) {
-                                        const uint32_t p = scur + mbcnt(m);
-                                        if (p < scap) {
-                                            a.slotIds[sbase + p] = cid[ch];
-                                            a.slotQ[sbase + p] = (uint8_t)qi;
-                                        }
-                                    }
-                                    scur += pm;
-                                }
-                                tot += pm;
-                            }
-                        }
-                    }
-                    if (PASS == PASS_FILL) {
-                        if (lane == qi) cur_l = cur;
-                    } else {
-                        if (lane == qi) cnt_l += tot;
-                    }
-                }
-            };
-            if (PASS == PASS_FILL && roomy) queries(std::true_type{});
-            else queries(std::false_type{});
+            // unswitched on the fill pass's capacity check where it pays
+            // (the single-join kernel); one copy with the check otherwise
+            if (UNSW && PASS == PASS_FILL && roomy) {
+#define GDSJ_RY true
+#include "join_query_loop.inc"
+#undef GDSJ_RY
+            } else {
+#define GDSJ_RY (UNSW ? false : roomy)
+#include "join_query_loop.inc"
+#undef GDSJ_RY
+            }
             if (STAMP) {
                 // the reverse direction of this chunk batch's hits
 #pragma unroll
