@@ -45,6 +45,12 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// add + number of set bits of m in lanes below this lane
+__device__ __forceinline__ uint32_t mbcnt_add(uint64_t m, uint32_t add) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, add));
+}
+
 __device__ __forceinline__ uint32_t popc64(uint64_t m) { return (uint32_t)__popcll(m); }
 
 // lanes 0 .. n-1 (all lanes for n >= 64)

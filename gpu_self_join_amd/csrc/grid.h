@@ -5,7 +5,8 @@
 // table and work tiles the join kernels consume.
 //
 // HBM layout after build (sorted = points ordered by (cell linear id, id)):
-//   xs        [D][ld] DTYPE  SoA coordinates in sorted order, dims permuted
+//   xs        [D][ld] DTYPE  SoA coordinates in sorted order, dims permuted;
+//                            positions n .. ld-1 (ld > n) hold NaN
 //   ids       [n]     uint32 sorted position -> original point id
 //   cellKey   [nc]    uint64 sorted unique linear ids (gridCellLookup)
 //   cellStart [nc+1]  uint32 sorted-position range of each cell (grid)

@@ -173,6 +173,10 @@ __global__ void k_soa(const T *__restrict__ data, uint32_t n, uint32_t d, const 
                       const uint32_t *__restrict__ order, T *__restrict__ xs, uint64_t ld,
                       uint32_t *__restrict__ ids, uint32_t *__restrict__ nonfinite) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    // positions n .. ld-1 (at least one) hold NaN: the join's lanes past the
+    // end of a candidate window read them, and NaN fails every compare
+    if ((uint64_t)s < ld - n)
+        for (uint32_t l = 0; l < d; l++) xs[(uint64_t)l * ld + n + s] = (T)__builtin_nan("");
     if (s >= n) return;
     uint32_t id = order[s];
     ids[s] = id;
@@ -400,7 +404,7 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     k_heads<<<ceil_div(n, 256), 256, 0, s>>>(skeys, n, flag);
     GDSJ_TRY(exclusive_scan_u32_u32(flag, cidx, n, true, G.tmp.p, s));
     // ---- SoA (sorted order, dims permuted), launched before the cell count sync
-    G.ld = ((uint64_t)n + 63) / 64 * 64;
+    G.ld = ((uint64_t)n + 1 + 63) / 64 * 64;  // n points + NaN padding
     GDSJ_TRY(G.xs.ensure((size_t)G.ld * d * sizeof(T) + 256));
     uint32_t *dperm = (uint32_t *)((char *)G.aux.p + (((size_t)mmBlocks * k * 2 * sizeof(T) + 255) & ~(size_t)255));
     GDSJ_HIP(hipMemcpyAsync(dperm, G.perm, d * 4, hipMemcpyHostToDevice, s));

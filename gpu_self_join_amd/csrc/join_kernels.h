@@ -196,7 +196,8 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
         for (int l = 0; l < (QREG ? D : 1); l++) qv[l] = lane < nq ? xs[(uint64_t)l * ld + q0 + lane] : (T)0;
     }
     uint32_t cnt_l = 0;
-    uint64_t cur_l = 0;
+    uint64_t cur_l = 0;  // fill pass: this lane's query's next CSR slot
+    uint32_t inc_l = 0;  // fill pass: its entries written in the current batch
     uint64_t work = 0;
     if (PASS == PASS_FILL && lane < nq) cur_l = a.offsets[qid_l];
     uint32_t scur = 0, scap = 0;
@@ -262,8 +263,11 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                 cpos[ch] = sp;
                 other[ch] = STAMP && valid[ch] && (sp < own0 || sp >= own1);
                 if (D > 0 && ch < (int)nch) {
+                    // lanes past the end read the NaN padding behind the last
+                    // point (position ld - 1), which fails every compare
+                    const uint64_t xp = valid[ch] ? (uint64_t)sp : ld - 1;
 #pragma unroll
-                    for (int l = 0; l < (D > 0 ? D : 1); l++) c[ch][l] = xs[(uint64_t)l * ld + sp];
+                    for (int l = 0; l < (D > 0 ? D : 1); l++) c[ch][l] = xs[(uint64_t)l * ld + xp];
                 }
                 if (NEED_IDS) cid[ch] = (ch < (int)nch) ? ids[sp] : 0u;
             }
@@ -278,6 +282,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
 #include "join_query_loop.inc"
 #undef GDSJ_RY
             }
+            if (PASS == PASS_FILL) cur_l += inc_l;
             if (STAMP) {
                 // the reverse direction of this chunk batch's hits
 #pragma unroll
