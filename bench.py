@@ -23,6 +23,24 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 / fp16 MFMA
 VALU_PEAK_TFLOPS = {4: 157.3, 8: 78.6}  # MI355X FP32 / FP64 (by DTYPE size)
+# VALU issue: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-instructions/s
+# (non-packed FP32 and FP64 alike; packed FP32 is not used)
+VALU_ISSUE_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def valu_ops_per_pair(d, ilp, shortcircuit):
+    """VALU lane-instructions of one candidate pair's distance and compare,
+    in the accumulation order the parameters select: SEQ = sub + FMA per
+    dim; ILP with short-circuit = sub + FMA for each group's first dim,
+    sub + mul + add for the others; ILP without = sub + FMA per dim plus
+    the fold of the ILP partials."""
+    if ilp == 0:
+        ops = 2 * d
+    elif shortcircuit:
+        ops = 3 * d - (d + ilp - 1) // ilp
+    else:
+        ops = 2 * d + min(ilp, d) - 1
+    return ops + 1
 
 
 def cpu_baseline(x, eps, params, sample_frac, seed=123):
@@ -340,8 +358,13 @@ def main():
         vflops = 3.0 * d * s["candidates"]
         vpeak = VALU_PEAK_TFLOPS[esz]
         vt = vflops / (kms[dom] * 1e-3) / 1e12
+        ops = valu_ops_per_pair(d, base["ilp"], base["shortcircuit"]) * float(s["candidates"])
+        it = ops / (kms[dom] * 1e-3) / 1e12
         roof["valu"] = {"achieved": round(vt, 3), "peak": vpeak, "unit": "TFLOP/s", "frac": round(vt / vpeak, 4),
-                        "alg_flops_per_launch": vflops, "candidate_pairs": s["candidates"]}
+                        "alg_flops_per_launch": vflops, "candidate_pairs": s["candidates"],
+                        "issue": {"achieved": round(it, 3), "peak": round(VALU_ISSUE_PEAK_TOPS, 2),
+                                  "unit": "T lane-ops/s", "frac": round(it / VALU_ISSUE_PEAK_TOPS, 4),
+                                  "ops_per_pair": valu_ops_per_pair(d, base["ilp"], base["shortcircuit"])}}
     if s["filter_rechecks"] > 0 and dom != "compact":
         # the MFMA Gram filter decides the pass: algorithmic work 2*D flops per
         # candidate pair (C = queries x candidates), against the dense

@@ -11,10 +11,11 @@
 //   cellKey   [nc]    uint64 sorted unique linear ids (gridCellLookup)
 //   cellStart [nc+1]  uint32 sorted-position range of each cell (grid)
 //   cellCoord [k][nc] uint32 cell coordinates per indexed dim
-//   hkeys/hvals [cap] open-addressing table: row (linear id without dim 0)
-//                     + 1 -> first cell of the row, or a direct table linear
-//                     id -> cell (replaces the binary searches of
-//                     kernel.cu:395-398)
+//   hkeys     [cap][2] uint64 open-addressing table, one 16-byte slot per
+//                     entry: {row (linear id without dim 0) + 1, first cell
+//                     of the row}; or
+//   hvals     [cells] uint32 a direct table linear id -> cell (small grids)
+//                     (both replace the binary searches of kernel.cu:395-398)
 //   tiles     [nt]    uint4  {cell, q0, q1, 0}: up to 64 queries of one cell
 #pragma once
 #include "common.h"
@@ -71,7 +72,8 @@ __device__ __forceinline__ uint32_t cell_hash(uint64_t key, uint32_t bits) {
 // linear id, 0xFFFFFFFF = empty.  Hashed (hbits > 0): the table holds rows
 // (cells equal in dims 1..k-1; their linear ids are row + c_0, so a row's
 // cells are contiguous in cell order): key row + 1 -> first cell of the row,
-// linear probing in a table at most half full (a miss ends at an empty slot).
+// linear probing in a table at most half full (a miss ends at an empty slot),
+// key and value side by side in 16-byte slots.
 // Returns the cell (direct) or the row's first cell (hashed), or -1.
 __device__ __forceinline__ int32_t cell_find(const uint64_t *__restrict__ hkeys, const uint32_t *__restrict__ hvals,
                                              uint32_t bits, uint64_t lin) {
@@ -80,9 +82,9 @@ __device__ __forceinline__ int32_t cell_find(const uint64_t *__restrict__ hkeys,
     uint32_t h = cell_hash(lin, bits);
     const uint64_t want = lin + 1;
     for (uint32_t probe = 0; probe <= mask; probe++) {
-        uint64_t k = hkeys[h];
-        if (k == want) return (int32_t)hvals[h];
-        if (k == 0) return -1;
+        const ulonglong2 e = reinterpret_cast<const ulonglong2 *>(hkeys)[h];  // key and cell in one load
+        if (e.x == want) return (int32_t)e.y;
+        if (e.x == 0) return -1;
         h = (h + 1) & mask;
     }
     return -1;

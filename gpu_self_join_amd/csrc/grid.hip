@@ -41,45 +41,89 @@ __global__ void k_sample_gather(const T *__restrict__ data, uint32_t n, uint32_t
     out[(uint64_t)l * m + j] = data[(uint64_t)j * 100u * d + l];
 }
 
-// one thread per dimension: the reference's sequential sums, unfused
+// One block of 256 threads per dimension: the reference's sequential sums,
+// unfused, in sample order, by lane 0 of wave 0.  The sample column streams
+// through LDS in chunks: waves 1..3 load chunk c + 1 (all of a thread's loads
+// issued before its stores) while lane 0 adds chunk c, so the dependent add
+// chain reads LDS instead of waiting on global memory.
+constexpr uint32_t VAR_LOADERS = 192;  // threads of waves 1..3
+constexpr uint32_t VAR_PER = 16;       // values per loader thread and chunk
+constexpr uint32_t VAR_CHUNK = VAR_LOADERS * VAR_PER;
+
 template <typename T>
-__global__ void k_variance(const T *__restrict__ samples, uint32_t m, uint32_t d, T denom,
-                           T *__restrict__ var) {
-    uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= d) return;
-    const T *col = samples + (uint64_t)l * m;
-    // loads are batched ahead of the dependent add chain (same order of adds)
-    constexpr uint32_t B = 32;
-    T sum = 0;
+__device__ __forceinline__ void var_stage(const T *__restrict__ col, uint32_t m, uint32_t base, uint32_t t,
+                                          uint32_t nthr, T *dst) {
+    T v[VAR_PER];
+#pragma unroll
+    for (uint32_t u = 0; u < VAR_PER; u++) {
+        const uint32_t i = t + u * nthr;
+        v[u] = (i < VAR_CHUNK && base + i < m) ? col[base + i] : (T)0;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < VAR_PER; u++) {
+        const uint32_t i = t + u * nthr;
+        if (i < VAR_CHUNK && base + i < m) dst[i] = v[u];
+    }
+}
+
+template <typename T, bool SQ>
+__device__ __forceinline__ T var_add(T sum, T v, T avg) {
+    if (SQ) {
+        const T t = v - avg;
+        const T q = t * t;
+        return sum + q;
+    }
+    return sum + v;
+}
+
+template <typename T, bool SQ>
+__device__ __forceinline__ T var_chunk_sum(const T *buf, uint32_t cnt, T sum, T avg) {
+    // 32 values per step, read (16-byte LDS loads) before the step's adds
+    constexpr uint32_t U = 32;
     uint32_t j = 0;
-    for (; j + B <= m; j += B) {
-        T v[B];
+    for (; j + U <= cnt; j += U) {
+        T v[U];
 #pragma unroll
-        for (uint32_t u = 0; u < B; u++) v[u] = col[j + u];
+        for (uint32_t u = 0; u < U; u++) v[u] = buf[j + u];
 #pragma unroll
-        for (uint32_t u = 0; u < B; u++) sum = sum + v[u];
+        for (uint32_t u = 0; u < U; u++) sum = var_add<T, SQ>(sum, v[u], avg);
     }
-    for (; j < m; j++) sum = sum + col[j];
-    const T avg = sum / denom;
-    sum = 0;
-    j = 0;
-    for (; j + B <= m; j += B) {
-        T v[B];
-#pragma unroll
-        for (uint32_t u = 0; u < B; u++) v[u] = col[j + u];
-#pragma unroll
-        for (uint32_t u = 0; u < B; u++) {
-            T t = v[u] - avg;
-            T sq = t * t;
-            sum = sum + sq;
+    for (; j < cnt; j++) sum = var_add<T, SQ>(sum, buf[j], avg);
+    return sum;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_variance(const T *__restrict__ samples, uint32_t m, T denom,
+                                                  T *__restrict__ var) {
+    __shared__ __align__(16) T buf[2][VAR_CHUNK];
+    __shared__ T s_avg;
+    const uint32_t l = blockIdx.x;
+    const T *col = samples + (uint64_t)l * m;
+    const uint32_t nchunk = (m + VAR_CHUNK - 1) / VAR_CHUNK;
+    T avg = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        T sum = 0;
+        if (threadIdx.x >= GDSJ_WAVE) var_stage<T>(col, m, 0u, threadIdx.x - GDSJ_WAVE, VAR_LOADERS, buf[0]);
+        __syncthreads();
+        for (uint32_t c = 0; c < nchunk; c++) {
+            const uint32_t base = c * VAR_CHUNK;
+            if (threadIdx.x >= GDSJ_WAVE) {
+                if (c + 1 < nchunk)
+                    var_stage<T>(col, m, base + VAR_CHUNK, threadIdx.x - GDSJ_WAVE, VAR_LOADERS, buf[(c + 1) & 1]);
+            } else if (threadIdx.x == 0) {
+                const uint32_t cnt = m - base < VAR_CHUNK ? m - base : VAR_CHUNK;
+                sum = pass == 0 ? var_chunk_sum<T, false>(buf[c & 1], cnt, sum, avg)
+                                : var_chunk_sum<T, true>(buf[c & 1], cnt, sum, avg);
+            }
+            __syncthreads();
         }
+        if (threadIdx.x == 0) {
+            if (pass == 0) s_avg = sum / denom;
+            else var[l] = sum / denom;
+        }
+        __syncthreads();
+        avg = s_avg;
     }
-    for (; j < m; j++) {
-        T t = col[j] - avg;
-        T sq = t * t;
-        sum = sum + sq;
-    }
-    var[l] = sum / denom;
 }
 
 // ------------------------------------------------------------ extent --
@@ -196,7 +240,7 @@ __global__ void k_soa(const T *__restrict__ data, uint32_t n, uint32_t d, const 
 // Rows: cells equal in dims 1..k-1 (linear id minus c_0); each row's first
 // cell is inserted under row + 1.
 __global__ void k_hash_build(const uint64_t *__restrict__ cellKey, uint32_t nc, uint32_t ncells0, uint32_t bits,
-                             unsigned long long *__restrict__ hkeys, uint32_t *__restrict__ hvals) {
+                             unsigned long long *__restrict__ hkeys) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const uint64_t row = cellKey[c] - cellKey[c] % ncells0;
@@ -205,9 +249,9 @@ __global__ void k_hash_build(const uint64_t *__restrict__ cellKey, uint32_t nc, 
     const unsigned long long key = (unsigned long long)row + 1ull;
     uint32_t h = cell_hash(row, bits);
     for (uint32_t probe = 0; probe <= mask; probe++) {
-        unsigned long long prev = atomicCAS(&hkeys[h], 0ull, key);
+        unsigned long long prev = atomicCAS(&hkeys[2 * (uint64_t)h], 0ull, key);
         if (prev == 0ull || prev == key) {
-            hvals[h] = c;
+            hkeys[2 * (uint64_t)h + 1] = c;
             return;
         }
         h = (h + 1) & mask;
@@ -328,7 +372,7 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         k_sample_gather<T><<<(uint32_t)((tot + 255) / 256), 256, 0, s>>>(data, n, d, m, samp);
         const T inv_sample = (T)(1.0 / (100 * 1.0));
         const T denom = (T)n * inv_sample;
-        k_variance<T><<<(d + 63) / 64, 64, 0, s>>>(samp, m, d, denom, var);
+        k_variance<T><<<d, 256, 0, s>>>(samp, m, denom, var);
         std::vector<T> hv(d);
         GDSJ_HIP(hipMemcpyAsync(hv.data(), var, d * sizeof(T), hipMemcpyDeviceToHost, s));
         GDSJ_HIP(hipStreamSynchronize(s));
@@ -445,11 +489,11 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         uint32_t hb = 6;
         while ((1ull << hb) < 2ull * nc) hb++;
         G.ga.hbits = hb;
-        GDSJ_TRY(G.hkeys.ensure(((size_t)1 << hb) * 8));
-        GDSJ_TRY(G.hvals.ensure(((size_t)1 << hb) * 4));
-        GDSJ_HIP(hipMemsetAsync(G.hkeys.p, 0, ((size_t)1 << hb) * 8, s));
+        GDSJ_TRY(G.hkeys.ensure(((size_t)1 << hb) * 16));
+        GDSJ_TRY(G.hvals.ensure(64));
+        GDSJ_HIP(hipMemsetAsync(G.hkeys.p, 0, ((size_t)1 << hb) * 16, s));
         k_hash_build<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellKey.as<uint64_t>(), nc, G.ga.nCells[0], hb,
-                                                        G.hkeys.as<unsigned long long>(), G.hvals.as<uint32_t>());
+                                                        G.hkeys.as<unsigned long long>());
     }
     k_cell_coords<<<ceil_div(nc, 256), 256, 0, s>>>(G.cellKey.as<uint64_t>(), nc, G.ga, G.cellCoord.as<uint32_t>());
     // pruning constants: a neighbor cell whose squared gap to every query of a

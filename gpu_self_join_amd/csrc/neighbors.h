@@ -66,11 +66,23 @@ __device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK
     if (x1 > cc[0] && gap + ghi[0] > ga.prune_lim) x1 = cc[0];
     if (stamp && top < 0 && !(cc[0] & 1u)) x0 = x1 = cc[0];
     if (ga.hbits) {
-        // one probe for the row, then the cells x0..x1 of it by key
+        // one probe for the row, then the cells x0..x1 of it by key: the
+        // row's first four keys are read together (sorted, so the number
+        // below a bound is its lower bound); longer rows continue galloping
         const int32_t r0 = cell_find(hkeys, hvals, ga.hbits, base);
         if (r0 < 0) return run;
-        const uint32_t f = key_lower_bound(cellKey, (uint32_t)r0, ga.ncell, base + x0);
-        const uint32_t e = key_lower_bound(cellKey, f, ga.ncell, base + x1 + 1);
+        const uint32_t r = (uint32_t)r0;
+        const uint64_t w0 = base + x0, w1 = base + x1 + 1;
+        uint32_t nf = 0, ne = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) {
+            const uint64_t key = r + i < ga.ncell ? cellKey[r + i] : ~0ull;
+            nf += key < w0 ? 1u : 0u;
+            ne += key < w1 ? 1u : 0u;
+        }
+        uint32_t f = r + nf, e = r + ne;
+        if (nf == 4) f = key_lower_bound(cellKey, r + 4, ga.ncell, w0);
+        if (ne == 4) e = key_lower_bound(cellKey, f, ga.ncell, w1);
         if (f < e) run = make_uint2(cellStart[f], cellStart[e]);
         return run;
     }

@@ -26,7 +26,10 @@ __global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uin
                                                     uint32_t *__restrict__ nruns, uint32_t *__restrict__ cellCand,
                                                     float *__restrict__ cellHist) {
     const uint32_t lane = lane_id();
-    const uint32_t cell = uniform((uint32_t)(blockIdx.x * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
+    // each XCD takes a contiguous range of cells: cells near in key order
+    // share neighbor rows, whose hash slots and keys then meet in one L2
+    const uint32_t lblock = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t cell = uniform((uint32_t)(lblock * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
     if (cell >= ga.ncell) return;
     __shared__ double s_gap[256 / GDSJ_WAVE][2][MAXK];
     double *glo = s_gap[threadIdx.x / GDSJ_WAVE][0];

@@ -99,6 +99,38 @@ def test_random_parity(sj, case):
     _check(sj, x, eps, k=k, ilp=ilp, sc=sc, reorder=reorder)
 
 
+def _reorder_data(n, d, dt, kind, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "ties":
+        # columns scaled apart by below an ulp: rounding decides the ranking,
+        # which then follows the exact order of the sums (the reference's
+        # sequential, unfused accumulation)
+        base = rng.standard_normal(n) * 3.0 + 5.0
+        sh = 25 if np.dtype(dt) == np.float32 else 54
+        x = np.stack([base * (1.0 + j * 2.0 ** -sh) for j in range(d)], axis=1)
+    elif kind == "equal":
+        # identical columns: equal variances, ties kept in dimension order
+        base = rng.random(n) * 10.0
+        x = np.stack([base] * d, axis=1)
+    else:
+        c = rng.random((6, d)) * 10.0
+        x = c[rng.integers(0, 6, n)] + rng.standard_normal((n, d)) * rng.random(d) * 2.0
+    return np.ascontiguousarray(x.astype(dt))
+
+
+# n / 100 samples per dimension: several staging chunks, ragged last chunk
+@pytest.mark.parametrize("n,d,dt,kind", [(450_123, 8, np.float32, "ties"), (1_000_000, 12, np.float32, "gmm"),
+                                         (333_333, 6, np.float64, "ties"), (205_000, 5, np.float32, "equal"),
+                                         (150, 4, np.float64, "gmm")])
+def test_reorder_permutation_equals_oracle(sj, n, d, dt, kind):
+    x = _reorder_data(n, d, dt, kind, n + d)
+    eps = float(np.asarray(1e-3, dtype=dt))
+    _, _, r = _gpu_join(sj, x, eps, k=2, reorder=1)
+    perm, _ = O.reorder_perm(x)
+    print("reorder n=%d d=%d %s perm %s" % (n, d, kind, list(perm)), flush=True)
+    assert list(r.stats["perm"][:d]) == [int(v) for v in perm]
+
+
 def test_dense_cell_many_tiles(sj):
     # one cell holds far more than 64 queries: tiles split the cell
     rng = np.random.default_rng(1)
