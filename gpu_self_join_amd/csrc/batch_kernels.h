@@ -80,22 +80,22 @@ __global__ __launch_bounds__(256) void k_batch_join(const JoinArgs *__restrict__
         if (SPEC) {
             switch (a.d) {
                 case 2:
-                    join_tile<T, 2, MODE, ILPV, PASS, BatchCMax<T, 2, BUDGET>::value, false, 0, false>(xs, a.ids, a, tile);
+                    join_tile<T, 2, MODE, ILPV, PASS, BatchCMax<T, 2, BUDGET>::value, false, 0, false, false>(xs, a.ids, a, tile);
                     continue;
                 case 4:
-                    join_tile<T, 4, MODE, ILPV, PASS, BatchCMax<T, 4, BUDGET>::value, false, 0, false>(xs, a.ids, a, tile);
+                    join_tile<T, 4, MODE, ILPV, PASS, BatchCMax<T, 4, BUDGET>::value, false, 0, false, false>(xs, a.ids, a, tile);
                     continue;
                 case 8:
-                    join_tile<T, 8, MODE, ILPV, PASS, BatchCMax<T, 8, BUDGET>::value, false, 0, false>(xs, a.ids, a, tile);
+                    join_tile<T, 8, MODE, ILPV, PASS, BatchCMax<T, 8, BUDGET>::value, false, 0, false, false>(xs, a.ids, a, tile);
                     continue;
                 case 16:
-                    join_tile<T, 16, MODE, ILPV, PASS, BatchCMax<T, 16, BUDGET>::value, false, 0, false>(xs, a.ids, a, tile);
+                    join_tile<T, 16, MODE, ILPV, PASS, BatchCMax<T, 16, BUDGET>::value, false, 0, false, false>(xs, a.ids, a, tile);
                     continue;
                 default:
                     break;
             }
         }
-        join_tile<T, 0, MODE, 0, PASS, 0, false, 0, false>(xs, a.ids, a, tile);
+        join_tile<T, 0, MODE, 0, PASS, 0, false, 0, false, false>(xs, a.ids, a, tile);
     }
 }
 

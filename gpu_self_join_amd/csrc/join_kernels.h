@@ -143,8 +143,10 @@ __device__ __forceinline__ T dist_mem(const T *__restrict__ xs, uint64_t ld, uin
 // QRW: most query coordinate words held in VGPRs (0: queries read from memory).
 // UNSW: two copies of the fill pass's query loop (with and without the
 // capacity check); a single copy keeps the batch kernel's registers down.
+// BMAP: run-start bitmap for the candidate positions of precomputed runs
+// (else the binary search over the window's runs only).
 template <typename T, int D, int MODE, int ILPV, int PASS, int CM = 0, bool STAMP = false, int QRW = 16,
-          bool UNSW = true>
+          bool UNSW = true, bool BMAP = true>
 __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
                                           const JoinArgs &a, const uint32_t tile) {
     constexpr int CMAX = CM > 0 ? CM : CMaxFor<T, D>::value;
@@ -180,6 +182,8 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
     }
     const uint32_t qid_l = lane < nq ? ids[q0 + lane] : 0u;
     __shared__ uint32_t s_qid[STAMP ? 256 / GDSJ_WAVE : 1][GDSJ_WAVE];
+    constexpr int BMW = 64;  // run-start bitmap words per wave
+    __shared__ uint64_t s_bm[BMAP ? 256 / GDSJ_WAVE : 1][BMAP ? BMW : 1];
     uint32_t own0 = 0, own1 = 0;  // sorted-position range of the tile's own cell
     if (STAMP) {
         own0 = uniform(a.cellStart[cell]);
@@ -238,8 +242,22 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
         }
         const uint32_t rexcl = incl - len;
         const uint32_t rstart = run.x;
+        // Precomputed runs are non-empty: a window of up to BMW * 64
+        // candidates marks its run starts in an LDS bitmap (bit rexcl - 1 of
+        // each run after the first), and a chunk's lanes find their run from
+        // the chunk's bitmap word instead of a binary search over the runs.
+        const bool bmap = BMAP && pre && total <= (uint32_t)(BMW * GDSJ_WAVE);
+        uint64_t *bm = s_bm[BMAP ? threadIdx.x / GDSJ_WAVE : 0];
+        if (bmap) {
+            if (lane < (total + GDSJ_WAVE - 1) / GDSJ_WAVE) bm[lane] = 0ull;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < nrw && rexcl > 0)
+                atomicOr(reinterpret_cast<uint32_t *>(bm) + ((rexcl - 1u) >> 5), 1u << ((rexcl - 1u) & 31u));
+            __builtin_amdgcn_wave_barrier();
+        }
+        const uint32_t delta = rstart - rexcl;  // position - window index, per run
         // a valid position for lanes past the end: first candidate of the window
-        const uint32_t fpos = uniform(cand_pos(0u, nrw, rstart, rexcl));
+        const uint32_t fpos = bmap ? uniform(shfl_u32(rstart, 0u)) : uniform(cand_pos(0u, nrw, rstart, rexcl));
 
         for (uint32_t cb = 0; cb < total; cb += GDSJ_WAVE * CMAX) {
             const uint32_t nch = uniform((total - cb + GDSJ_WAVE - 1) / GDSJ_WAVE < (uint32_t)CMAX
@@ -258,7 +276,19 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                 valid[ch] = g < total;
                 if (STAMP && PASS == PASS_COUNT) jc[ch] = 0;
                 if (STAMP && PASS == PASS_FILL) jm[ch] = 0;
-                uint32_t sp = cand_pos(valid[ch] ? g : 0u, nrw, rstart, rexcl);
+                uint32_t sp = fpos;
+                if (bmap) {
+                    if (ch < (int)nch) {
+                        // run of g = (runs starting at or before the chunk's
+                        // first index) + (starts in (base, g]) - 1
+                        const uint32_t base = cb + (uint32_t)ch * GDSJ_WAVE;
+                        const uint64_t mw = uniform64(bm[base >> 6]);
+                        const uint32_t le = popc64(ballot(rexcl <= base));
+                        sp = g + shfl_u32(delta, mbcnt_add(mw, le - 1u));
+                    }
+                } else {
+                    sp = cand_pos(valid[ch] ? g : 0u, nrw, rstart, rexcl);
+                }
                 sp = valid[ch] ? sp : fpos;
                 cpos[ch] = sp;
                 other[ch] = STAMP && valid[ch] && (sp < own0 || sp >= own1);
