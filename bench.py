@@ -182,6 +182,23 @@ def run_variant(sj, pts, eps, kw, steps, warmup, total=None):
     return total, 1e3 * wall / steps, per
 
 
+def host_path(x, eps, kw, steps=2, warmup=1):
+    """The drop-in's host-buffer path (gdsjoin): host points in, the counts
+    and the table copied back over PCIe, the library's per-device handle
+    reused across calls.  Reported beside value, never as it."""
+    from gpu_self_join_amd import _lib, gdsjoin
+    p = _lib.default_params(**kw)
+    dt = "float" if x.dtype == np.float32 else "double"
+    for _ in range(warmup):
+        gdsjoin(x, eps, x.shape[1], dt, params=p)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        counts, table = gdsjoin(x, eps, x.shape[1], dt, params=p)
+    wall = (time.perf_counter() - t0) / steps
+    return {"value": len(table) / wall, "unit": "pairs/s", "ms_per_join": round(1e3 * wall, 3), "steps": steps,
+            "path": "gdsjoin(): host array in, counts and table out over PCIe"}
+
+
 def bench_batched(args):
     """The batched workload: 256 instances in one job (gdsj_batch_*)."""
     import torch
@@ -269,6 +286,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="gmm6d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-host-path", action="store_true", help="skip timing the PCIe-inclusive gdsjoin() path")
     ap.add_argument("--cpu-sample", type=float, default=0.01)
     ap.add_argument("--variants", default=None,
                     help="estimators to time (exact = counting pass + fill; learned = MLP-sized single pass)")
@@ -310,9 +328,11 @@ def main():
     if args.variants is None:
         args.variants = "exact,nostamp,learned" if base.get("stamp") else "exact,learned"
     results = {}
+    kws = {}
     total = None
     for vname in args.variants.split(","):
         kw = dict(base, estimator=EST[vname])
+        kws[vname] = kw
         sj = SelfJoin(0)
         steps, warmup = args.steps, args.warmup
         if vname == "learned":
@@ -411,6 +431,7 @@ def main():
                                                             "ms_fill")) * 1e-3),
         "filter_error": filter_error(pts, eps, base) if d >= 16 else None,
         "roofline": roof,
+        "host_path": None if args.no_host_path else host_path(x, eps, kws[best]),
     }
     if args.no_cpu_baseline:
         out["cpu_baseline"] = None

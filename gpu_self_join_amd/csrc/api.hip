@@ -622,12 +622,14 @@ int gdsj_model_set(gdsj_handle *h, const float *params, uint32_t count, double a
 // ---------------------------------------------------------------- legacy --
 // The reference keeps the pending table in a process-global pointer
 // (main.cu:359) until copyResultIntoPythonArray copies and frees it
-// (main.cu:507-511).  Here it is a mutex-protected global as well.
+// (main.cu:507-511).  Here the pending table stays in the device's legacy
+// handle (a mutex-protected global) and is copied from HBM straight into the
+// caller's array, with no host staging copy.
 static std::mutex g_legacy_mu;
 static gdsj_params g_legacy_params;
 static bool g_legacy_params_set = false;
 static gdsj_handle *g_legacy_handle[64] = {};
-static uint32_t *g_pending = nullptr;
+static gdsj_handle *g_pending = nullptr;  // handle holding the pending table
 static uint64_t g_pending_n = 0;
 static int g_legacy_status = GDSJ_OK;
 
@@ -648,7 +650,6 @@ static void legacy_join(const void *data, unsigned int n, double eps, unsigned i
                         unsigned int *counts) {
     std::lock_guard<std::mutex> lk(g_legacy_mu);
     if (counts && n) memset(counts, 0, (size_t)n * 4);
-    free(g_pending);
     g_pending = nullptr;
     g_pending_n = 0;
     g_legacy_status = GDSJ_OK;
@@ -674,18 +675,7 @@ static void legacy_join(const void *data, unsigned int n, double eps, unsigned i
         g_legacy_status = rc;
         return;
     }
-    g_pending = (uint32_t *)malloc((size_t)total * 4 + 4);
-    if (!g_pending) {
-        g_legacy_status = GDSJ_EOOM;
-        return;
-    }
-    rc = gdsj_result_copy(h, g_pending, total);
-    if (rc) {
-        free(g_pending);
-        g_pending = nullptr;
-        g_legacy_status = rc;
-        return;
-    }
+    g_pending = h;
     g_pending_n = total;
 }
 
@@ -703,8 +693,15 @@ void copyResultIntoPythonArray(unsigned int *outPythonNeighborTable, unsigned in
     std::lock_guard<std::mutex> lk(g_legacy_mu);
     if (!g_pending) return;
     uint64_t m = numResults < g_pending_n ? numResults : g_pending_n;
-    if (outPythonNeighborTable && m) memcpy(outPythonNeighborTable, g_pending, (size_t)m * 4);
-    free(g_pending);
+    if (outPythonNeighborTable && m) {
+        gdsj_handle *h = g_pending;
+        hipError_t e = hipSetDevice(h->device);
+        if (e == hipSuccess) e = hipMemcpy(outPythonNeighborTable, h->ids.p, (size_t)m * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            set_error("copyResultIntoPythonArray: %s", hipGetErrorString(e));
+            g_legacy_status = GDSJ_EDEVICE;
+        }
+    }
     g_pending = nullptr;
     g_pending_n = 0;
 }

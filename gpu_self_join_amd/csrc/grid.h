@@ -81,8 +81,23 @@ __device__ __forceinline__ int32_t cell_find(const uint64_t *__restrict__ hkeys,
     const uint32_t mask = (1u << bits) - 1u;
     uint32_t h = cell_hash(lin, bits);
     const uint64_t want = lin + 1;
+    if (bits <= 27) {
+        // key and cell in one 16-byte buffer load (the compiler would
+        // otherwise fetch the cell by a second, dependent load after the
+        // key compare); slot offsets fit the 32-bit buffer offset
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)hkeys, (short)0, (int)((mask + 1u) * 16u), 0x00020000);
+        for (uint32_t probe = 0; probe <= mask; probe++) {
+            const auto e = __builtin_amdgcn_raw_buffer_load_b128(rs, h * 16u, 0, 0);
+            const uint64_t key = ((uint64_t)e[1] << 32) | e[0];
+            if (key == want) return (int32_t)e[2];
+            if (key == 0) return -1;
+            h = (h + 1) & mask;
+        }
+        return -1;
+    }
     for (uint32_t probe = 0; probe <= mask; probe++) {
-        const ulonglong2 e = reinterpret_cast<const ulonglong2 *>(hkeys)[h];  // key and cell in one load
+        const ulonglong2 e = reinterpret_cast<const ulonglong2 *>(hkeys)[h];
         if (e.x == want) return (int32_t)e.y;
         if (e.x == 0) return -1;
         h = (h + 1) & mask;

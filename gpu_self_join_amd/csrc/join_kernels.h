@@ -169,7 +169,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
         nwin = uniform(a.nruns[cell]);
     } else {
         nwin = a.ncombo;
-        for (uint32_t j = 0; j < ga.k; j++) cc[j] = uniform(a.cellCoord[(uint64_t)j * ga.ncell + cell]);
+        load_cell_coords(a.cellCoord, ga.ncell, cell, ga.k, cc);
         if (a.prune) {
             tile_gaps<T>(xs, ld, q0, nq, cc, ga, glo, ghi);
         } else {

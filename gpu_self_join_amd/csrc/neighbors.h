@@ -98,6 +98,17 @@ __device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK
     return run;
 }
 
+// Cell coordinates of a cell in the k indexed dims, all loads issued
+// together (entries past k repeat dim 0 and are not used).
+__device__ __forceinline__ void load_cell_coords(const uint32_t *__restrict__ cellCoord, uint32_t ncell,
+                                                 uint32_t cell, uint32_t k, uint32_t (&cc)[MAXK]) {
+#pragma unroll
+    for (int j = 0; j < MAXK; j++) {
+        const uint32_t jj = (uint32_t)j < k ? (uint32_t)j : 0u;
+        cc[j] = uniform(cellCoord[(uint64_t)jj * ncell + cell]);
+    }
+}
+
 // Squared gaps from the bounding box of the n query points at sorted
 // positions [q0, q0+n) to the faces of their cell in each indexed dim:
 // glo[j] towards coordinate c_j - 1, ghi[j] towards c_j + 1.  Face
@@ -109,6 +120,42 @@ __device__ __forceinline__ void tile_gaps(const T *__restrict__ xs, uint64_t ld,
                                           const uint32_t (&cc)[MAXK], const GridArgs &ga, double *glo,
                                           double *ghi) {
     const uint32_t lane = lane_id();
+    if (n <= (uint32_t)GDSJ_WAVE) {
+        // one point per lane: every dim's loads are issued before the first
+        // reduction (dims past k read dim 0 again and are not used)
+        // (lanes past n read point 0 of the range, n >= 1, and are masked below)
+        const uint32_t pl = lane < n ? lane : 0u;
+        T raw[MAXK];
+#pragma unroll
+        for (int j = 0; j < MAXK; j++) {
+            const uint32_t jj = (uint32_t)j < ga.k ? (uint32_t)j : 0u;
+            raw[j] = xs[(uint64_t)jj * ld + q0 + pl];
+        }
+        double v[MAXK];
+#pragma unroll
+        for (int j = 0; j < MAXK; j++) v[j] = (double)raw[j];
+#pragma unroll
+        for (int j = 0; j < MAXK; j++) {
+            if ((uint32_t)j >= ga.k) break;
+            double mn = lane < n ? v[j] : 1e300, mx = lane < n ? v[j] : -1e300;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                mn = fmin(mn, __shfl_xor(mn, off, GDSJ_WAVE));
+                mx = fmax(mx, __shfl_xor(mx, off, GDSJ_WAVE));
+            }
+            const double lo_face = ga.minArr[j] + (double)cc[j] * ga.eps;
+            const double hi_face = lo_face + ga.eps;
+            const double tol = 8.0 * ga.ulp * (fabs(ga.minArr[j]) + fabs(hi_face) + 2.0 * ga.eps);
+            const double gl = fmax(0.0, mn - lo_face - tol);
+            const double gh = fmax(0.0, hi_face - tol - mx);
+            if (lane == 0) {
+                glo[j] = gl * gl;
+                ghi[j] = gh * gh;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        return;
+    }
     for (uint32_t j = 0; j < ga.k; j++) {
         double mn = 1e300, mx = -1e300;
         for (uint32_t b = 0; b < n; b += GDSJ_WAVE) {

@@ -15,7 +15,7 @@
 namespace gdsj {
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uint64_t ld, GridArgs ga,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cell_runs(const T *__restrict__ xs, uint64_t ld, GridArgs ga,
                                                     const uint32_t *__restrict__ cellStart,
                                                     const uint32_t *__restrict__ cellCoord,
                                                     const uint64_t *__restrict__ cellKey,
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void k_cell_runs(const T *__restrict__ xs, uin
     double *glo = s_gap[threadIdx.x / GDSJ_WAVE][0];
     double *ghi = s_gap[threadIdx.x / GDSJ_WAVE][1];
     uint32_t cc[MAXK];
-    for (uint32_t j = 0; j < ga.k; j++) cc[j] = uniform(cellCoord[(uint64_t)j * ga.ncell + cell]);
+    load_cell_coords(cellCoord, ga.ncell, cell, ga.k, cc);
     const uint32_t s0 = uniform(cellStart[cell]), s1 = uniform(cellStart[cell + 1]);
     if (prune) {
         tile_gaps<T>(xs, ld, s0, s1 - s0, cc, ga, glo, ghi);

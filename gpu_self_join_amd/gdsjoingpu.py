@@ -61,7 +61,12 @@ def gdsjoinmain(DATASET, EPSILON, NDIM, DTYPE, params=None, gpu=0):
     if NDIM <= 0:
         raise ValueError("NDIM must be positive")
     npdt = np.float32 if DTYPE == "float" else np.float64
-    data = np.ascontiguousarray(np.asarray(DATASET, dtype=np.float64).ravel().astype(npdt))
+    src = np.asarray(DATASET)
+    if src.dtype in (np.float32, np.float64):
+        # floating input: a direct cast is the same as through float64
+        data = np.ascontiguousarray(src.ravel(), dtype=npdt)
+    else:
+        data = np.ascontiguousarray(np.asarray(src, dtype=np.float64).ravel().astype(npdt))
     if data.size % NDIM:
         raise ValueError("len(DATASET)=%d is not a multiple of NDIM=%d" % (data.size, NDIM))
     num_points = data.size // NDIM
@@ -81,6 +86,9 @@ def gdsjoinmain(DATASET, EPSILON, NDIM, DTYPE, params=None, gpu=0):
     total = int(counts.sum(dtype=np.uint64))
     if total >= 2 ** 32:
         raise _lib.GdsjError(_lib.GDSJ_ERANGE, "table of %d ids exceeds the uint32 count of the C ABI" % total)
-    table = np.zeros(total, dtype=np.uint32)
+    table = np.empty(total, dtype=np.uint32)
     L.copyResultIntoPythonArray(table.ctypes.data, total)
+    rc = L.gdsj_legacy_status()
+    if rc != _lib.GDSJ_OK:
+        raise _lib.GdsjError(rc, _lib.last_error())
     return counts, table

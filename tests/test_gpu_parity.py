@@ -203,6 +203,25 @@ def test_sorted_rows_option(sj):
     assert np.array_equal(ids, ref["ids"])  # sorted rows are exactly the oracle's CSR
 
 
+@pytest.mark.parametrize("kind", ["float64_as_float", "int_list", "float32_2d"])
+def test_gdsjoin_input_conversion(gpu, kind):
+    # DATASET values go through float64 into DTYPE, as the reference's
+    # np.asfarray + ctypes conversion does; floating arrays are cast directly
+    rng = np.random.default_rng(12)
+    x64 = rng.random((2500, 3)) * 10
+    if kind == "float64_as_float":
+        data, ref_x = x64, x64.astype(np.float32)
+    elif kind == "int_list":
+        xi = rng.integers(0, 40, (2500, 3))
+        data, ref_x = xi.ravel().tolist(), xi.astype(np.float64).astype(np.float32)
+    else:
+        data, ref_x = x64.astype(np.float32), x64.astype(np.float32)
+    p = _lib.default_params(k=3, ilp=0, shortcircuit=0, reorder=0)
+    counts, table = gdsjoin(data, 1.5, 3, "float", params=p)
+    ref = O.join(np.ascontiguousarray(ref_x), float(np.float32(1.5)), k=3)
+    assert_csr_equal(counts, table, ref["counts"], ref["ids"])
+
+
 def test_legacy_symbols_two_call_protocol(gpu):
     L = _lib.load()
     rng = np.random.default_rng(8)
@@ -216,6 +235,17 @@ def test_legacy_symbols_two_call_protocol(gpu):
     L.copyResultIntoPythonArray(table.ctypes.data, len(table))  # second call is a no-op
     ref = O.join(x, 0.7, k=3)
     assert_csr_equal(counts, table, ref["counts"], ref["ids"])
+    # the pending table is held until copied: a shorter buffer takes its prefix
+    L.gdsj_set_legacy_params(C.byref(_lib.default_params(k=3, ilp=0, shortcircuit=0, reorder=0, stamp=0)))
+    L.GDSJoinPy(x.ctypes.data, 3000, C.c_float(0.7), 3, 0, counts.ctypes.data)
+    full = np.zeros(int(counts.sum()), np.uint32)
+    L.copyResultIntoPythonArray(full.ctypes.data, len(full))
+    L.GDSJoinPy(x.ctypes.data, 3000, C.c_float(0.7), 3, 0, counts.ctypes.data)
+    part = np.zeros(10, np.uint32)
+    L.copyResultIntoPythonArray(part.ctypes.data, 10)
+    assert L.gdsj_legacy_status() == 0
+    assert np.array_equal(part, full[:10])
+    assert_csr_equal(counts, full, ref["counts"], ref["ids"])
     L.gdsj_set_legacy_params(None)
 
 
