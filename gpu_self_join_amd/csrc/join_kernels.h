@@ -129,6 +129,61 @@ __device__ __forceinline__ T dist_mem(const T *__restrict__ xs, uint64_t ld, uin
     return acc;
 }
 
+// dist_mem with the coordinates of up to 8 dims loaded before their
+// arithmetic (16 loads in flight instead of a load round trip per dim); the
+// floating-point operations and their order are dist_mem's.  Used by the
+// Gram filter's exact rechecks, where each lane checks a different pair.
+template <typename T>
+__device__ __forceinline__ T dist_mem8(const T *__restrict__ xs, uint64_t ld, uint32_t qpos, uint32_t cpos,
+                                       uint32_t d, int mode, uint32_t ilp) {
+    constexpr uint32_t C = 8;
+    if (mode != ACC_ILP_NOSC) {
+        // SEQ: an FMA per dim.  ILP_SC: an FMA for the first dim of each
+        // group of ilp dims, a rounded product and an add for the others.
+        T acc = 0;
+        uint32_t g = 0;  // position of the dim within its group
+        for (uint32_t l0 = 0; l0 < d; l0 += C) {
+            T tv[C];
+#pragma unroll
+            for (uint32_t u = 0; u < C; u++) {
+                const uint64_t l = l0 + u < d ? l0 + u : l0;  // in range; unused past d
+                tv[u] = xs[l * ld + cpos] - xs[l * ld + qpos];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < C; u++) {
+                if (l0 + u < d) {
+                    if (mode == ACC_SEQ || g == 0) {
+                        acc = fma_rn(tv[u], tv[u], acc);
+                    } else {
+                        const T p = tv[u] * tv[u];
+                        acc = acc + p;
+                    }
+                    g = g + 1 == ilp ? 0u : g + 1;
+                }
+            }
+        }
+        return acc;
+    }
+    // ACC_ILP_NOSC: partial j accumulates dims j, j+ilp, ...; fold in j order
+    T acc = 0;
+    for (uint32_t j = 0; j < ilp; j++) {
+        T rj = 0;
+        for (uint32_t l0 = j; l0 < d; l0 += C * ilp) {
+            T tv[C];
+#pragma unroll
+            for (uint32_t u = 0; u < C; u++) {
+                const uint64_t l = l0 + u * ilp < d ? l0 + u * ilp : l0;
+                tv[u] = xs[l * ld + cpos] - xs[l * ld + qpos];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < C; u++)
+                if (l0 + u * ilp < d) rj = fma_rn(tv[u], tv[u], rj);
+        }
+        acc = (j == 0) ? rj : acc + rj;
+    }
+    return acc;
+}
+
 // One work tile of the join (one wave): the body shared by the per-join
 // kernel below and the persistent batched kernel (batch.hip).
 // CM: candidate chunks held in VGPRs (0: CMaxFor's default).
