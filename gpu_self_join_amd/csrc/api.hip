@@ -614,6 +614,7 @@ int gdsj_model_set(gdsj_handle *h, const float *params, uint32_t count, double a
     GDSJ_HIP(hipMemset(h->est.adam_m.p, 0, MLP_P * 4));
     GDSJ_HIP(hipMemset(h->est.adam_v.p, 0, MLP_P * 4));
     h->est.alpha = alpha;
+    h->est.alpha_heavy = alpha;
     h->est.init = h->est.trained = true;
     h->est.step = 0;
     return GDSJ_OK;

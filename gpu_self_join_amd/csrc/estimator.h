@@ -30,6 +30,7 @@ struct Estimator {
     bool trained = false;  // trained on at least one join
     uint32_t step = 0;     // Adam step
     double alpha = 0.25;   // slot headroom: cap = pred * (1 + alpha) + beta
+    double alpha_heavy = 0.25;  // headroom of tiles with many candidate pairs
     void release();
 };
 

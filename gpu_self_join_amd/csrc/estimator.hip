@@ -374,9 +374,11 @@ int estimator_train_rows(Estimator &E, const float *feat, const float *target, u
         const double y = exp2((double)ht[i]) - 1.0, p = std::max(exp2((double)hp[i]) - 1.0, 1.0);
         ratio[i] = y / p;
     }
+    const double rmax = *std::max_element(ratio.begin(), ratio.end());
     const size_t q = (size_t)(0.99 * (m - 1));
     std::nth_element(ratio.begin(), ratio.begin() + q, ratio.end());
     E.alpha = std::min(4.0, std::max(0.05, ratio[q] - 1.0));
+    E.alpha_heavy = std::min(16.0, std::max(E.alpha, rmax - 1.0));
     E.trained = true;
     float t = 0;
     (void)hipEventElapsedTime(&t, e0, e1);
@@ -407,9 +409,15 @@ __device__ __forceinline__ V block_reduce(V v) {
 constexpr uint32_t RED_BLOCK = 1024;
 
 // ------------------------------------------------------------- capacities --
+// Slot capacity per tile: pred (1 + alpha) + beta, between nq and every
+// candidate a hit.  Heavy tiles (at least HEAVY_PAIRS candidate pairs) take
+// alpha_heavy: redoing one of them after an overflow costs a full pass over
+// its candidates, while unused slots cost only memory.
+constexpr double HEAVY_PAIRS = 1048576.0;
+
 __global__ void k_capacity(const float *__restrict__ predLog2, const uint4 *__restrict__ tiles, uint32_t nt,
-                           const uint32_t *__restrict__ cellCand, float alpha, float beta, double perQuery,
-                           uint32_t *__restrict__ cap, unsigned long long *__restrict__ predSum,
+                           const uint32_t *__restrict__ cellCand, float alpha, float alpha_heavy, float beta,
+                           double perQuery, uint32_t *__restrict__ cap, unsigned long long *__restrict__ predSum,
                            float *__restrict__ predOut) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t mine = 0;
@@ -418,7 +426,8 @@ __global__ void k_capacity(const float *__restrict__ predLog2, const uint4 *__re
         const double nq = (double)(tl.z - tl.y);
         const double upper = nq * (double)cellCand[tl.x];  // every candidate a hit
         const double pred = predLog2 ? fmax(exp2((double)predLog2[t]) - 1.0, 0.0) : nq * perQuery;
-        double c = ceil(pred * (1.0 + alpha)) + beta;
+        const double al = upper >= HEAVY_PAIRS ? (double)alpha_heavy : (double)alpha;
+        double c = ceil(pred * (1.0 + al)) + beta;
         c = fmin(fmax(c, nq), upper);
         cap[t] = (uint32_t)c;
         mine = (uint64_t)pred;
@@ -439,7 +448,8 @@ int estimator_capacity_learned(Estimator &E, const Grid &G, uint32_t *capOut, ui
     GDSJ_HIP(hipMemsetAsync(E.work.p, 0, 8, s));
     if (nt)
         k_capacity<<<ceil_div(nt, RED_BLOCK), RED_BLOCK, 0, s>>>(E.pred.as<float>(), G.tiles.as<uint4>(), nt,
-                                                     G.cellCand.as<uint32_t>(), (float)E.alpha, 8.0f, 0.0, capOut,
+                                                     G.cellCand.as<uint32_t>(), (float)E.alpha,
+                                                     (float)E.alpha_heavy, 8.0f, 0.0, capOut,
                                                      E.work.as<unsigned long long>(), nullptr);
     GDSJ_HIP(hipGetLastError());
     GDSJ_HIP(hipMemcpyAsync(predicted, E.work.p, 8, hipMemcpyDeviceToHost, s));
@@ -506,7 +516,7 @@ int estimator_capacity_sampling(const Grid &G, const void *joinArgs, int mode, u
     GDSJ_HIP(hipMemsetAsync(sum, 0, 8, s));
     if (nt)
         k_capacity<<<ceil_div(nt, RED_BLOCK), RED_BLOCK, 0, s>>>(nullptr, G.tiles.as<uint4>(), nt, G.cellCand.as<uint32_t>(),
-                                                     (float)alpha, 8.0f, n ? (double)est / n : 0.0, capOut, sum,
+                                                     (float)alpha, (float)alpha, 8.0f, n ? (double)est / n : 0.0, capOut, sum,
                                                      predLog2Out);
     GDSJ_HIP(hipGetLastError());
     return GDSJ_OK;

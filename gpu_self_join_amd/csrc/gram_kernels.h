@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     __shared__ unsigned long long s_cur[256 / GDSJ_WAVE][TQ];
     __shared__ uint32_t s_hm[256 / GDSJ_WAVE][TQ];  // hit columns per row, current block
     __shared__ uint32_t s_cid[256 / GDSJ_WAVE][32];  // ids of the current block's columns
-    constexpr uint32_t GQ = 512;                        // recheck queue entries per wave
+    constexpr uint32_t GQ = TQ * 32;                    // recheck queue: a block's every pair
     __shared__ uint16_t s_q[256 / GDSJ_WAVE][GQ];       // survivors: row << 5 | column
     double *glo = s_gap[w][0];
     double *ghi = s_gap[w][1];
@@ -217,9 +217,8 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                 }
             }
             // exact rechecks of the block's survivors; hits set column bits of
-            // their row.  Up to GQ survivors are queued in LDS (row, column)
-            // and checked by all lanes at once, a pair per lane; more fall
-            // back to each lane checking its own survivors in turn.
+            // their row.  The survivors are queued in LDS (row, column) and
+            // checked by all lanes at once, a pair per lane.
             uint32_t ns = 0;
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) ns += __popc(smv[rb]);
@@ -232,55 +231,35 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
             const uint32_t stot = uniform(shfl_u32(sincl, GDSJ_WAVE - 1));
             if (stot == 0) continue;
             bool anyhit = false;
-            if (stot <= GQ) {
-                uint16_t *qq = s_q[w];
-                uint32_t o = sincl - ns;
+            uint16_t *qq = s_q[w];
+            uint32_t o = sincl - ns;
 #pragma unroll
-                for (int rb = 0; rb < RB; rb++) {
-                    uint32_t sm = smv[rb];
-                    while (sm) {
-                        const int i = __builtin_ctz(sm);
-                        sm &= sm - 1;
-                        const uint32_t r = rb * 32 + (i / 4) * 8 + half * 4 + (i % 4);
-                        qq[o++] = (uint16_t)((r << 5) | row_l);
-                    }
+            for (int rb = 0; rb < RB; rb++) {
+                uint32_t sm = smv[rb];
+                while (sm) {
+                    const int i = __builtin_ctz(sm);
+                    sm &= sm - 1;
+                    const uint32_t r = rb * 32 + (i / 4) * 8 + half * 4 + (i % 4);
+                    qq[o++] = (uint16_t)((r << 5) | row_l);
                 }
-                __builtin_amdgcn_wave_barrier();
-                for (uint32_t e0 = 0; e0 < stot; e0 += GDSJ_WAVE) {
-                    const uint32_t e = e0 + lane;
-                    const uint32_t v = e < stot ? (uint32_t)qq[e] : 0u;
-                    const uint32_t r = v >> 5, col = v & 31u;
-                    const uint32_t spc = shfl_u32(sp, col);  // lane col holds that candidate
-                    if (e < stot) {
-                        nrech++;
-                        const T dv = dist_mem8<T>(xs, ld, q0 + r, spc, a.d, mode, a.ilp);
-                        if (dv <= thr) {
-                            nhit++;
-                            anyhit = true;
-                            atomicOr(&s_hm[w][r], 1u << col);
-                        }
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-            } else {
-#pragma unroll
-                for (int rb = 0; rb < RB; rb++) {
-                    uint32_t sm = smv[rb];
-#pragma unroll 1
-                    while (sm) {
-                        const int i = __builtin_ctz(sm);
-                        sm &= sm - 1;
-                        const uint32_t r = rb * 32 + (i / 4) * 8 + half * 4 + (i % 4);
-                        nrech++;
-                        const T dv = dist_mem8<T>(xs, ld, q0 + r, sp, a.d, mode, a.ilp);
-                        if (dv <= thr) {
-                            nhit++;
-                            anyhit = true;
-                            atomicOr(&s_hm[w][r], 1u << row_l);
-                        }
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t e0 = 0; e0 < stot; e0 += GDSJ_WAVE) {
+                const uint32_t e = e0 + lane;
+                const uint32_t v = e < stot ? (uint32_t)qq[e] : 0u;
+                const uint32_t r = v >> 5, col = v & 31u;
+                const uint32_t spc = shfl_u32(sp, col);  // lane col holds that candidate
+                if (e < stot) {
+                    nrech++;
+                    const T dv = dist_mem8<T>(xs, ld, q0 + r, spc, a.d, mode, a.ilp);
+                    if (dv <= thr) {
+                        nhit++;
+                        anyhit = true;
+                        atomicOr(&s_hm[w][r], 1u << col);
                     }
                 }
             }
+            __builtin_amdgcn_wave_barrier();
             if (ballot(anyhit) == 0) continue;
             if (half == 0) s_cid[w][row_l] = ids[sp];
             __builtin_amdgcn_wave_barrier();
