@@ -88,6 +88,7 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     __shared__ unsigned long long s_cur[256 / GDSJ_WAVE][TQ];
     __shared__ uint32_t s_hm[256 / GDSJ_WAVE][TQ];  // hit columns per row, current block
     __shared__ uint32_t s_cid[256 / GDSJ_WAVE][32];  // ids of the current block's columns
+    __shared__ __align__(16) float s_wq[256 / GDSJ_WAVE][TQ];  // query weights
     constexpr uint32_t GQ = TQ * 32;                    // recheck queue: a block's every pair
     __shared__ uint16_t s_q[256 / GDSJ_WAVE][GQ];       // survivors: row << 5 | column
     double *glo = s_gap[w][0];
@@ -116,9 +117,12 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     }
     __builtin_amdgcn_wave_barrier();
 
+    // query weights (INFINITY past nq) in LDS: row r at s_wq[w][r]; the
+    // rows a lane's accumulator entries 4j..4j+3 cover are consecutive
+    for (uint32_t r = lane; r < TQ; r += GDSJ_WAVE) s_wq[w][r] = r < nq ? W[q0 + r] : INFINITY;
+    __builtin_amdgcn_wave_barrier();
     vec A[RB][KS];
     vec Al[RB][KS];
-    float wq[RB][16];
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
         const uint32_t qr = rb * 32 + row_l;
@@ -131,11 +135,6 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                 A[rb][ks] = (vec){};
                 if (SPLIT) Al[rb][ks] = (vec){};
             }
-        }
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint32_t r = rb * 32 + (i / 4) * 8 + half * 4 + (i % 4);
-            wq[rb][i] = r < nq ? W[q0 + r] : INFINITY;
         }
     }
     uint32_t scur = 0, scap = 0;
@@ -169,18 +168,35 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
         const uint32_t rexcl = incl - len;
         const uint32_t rstart = run.x;
         const uint32_t fpos = uniform(cand_pos(0u, nrw, rstart, rexcl));
+        // block cb + 32's position and operands are loaded while block cb's
+        // MFMAs and rechecks run
+        uint32_t sp_n;
+        vec Bn[KS], Bln[KS];
+        float wc_n;
+        auto load_block = [&](uint32_t cbl) {
+            const uint32_t gl = cbl + row_l;
+            const bool vl = gl < total;
+            uint32_t p = cand_pos(vl ? gl : 0u, nrw, rstart, rexcl);
+            p = vl ? p : fpos;
+            sp_n = p;
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                Bn[ks] = load8<LP>(H + (uint64_t)p * DP + ks * 16 + half * 8);
+                if (SPLIT) Bln[ks] = load8<LP>(Lo + (uint64_t)p * DP + ks * 16 + half * 8);
+            }
+            wc_n = vl ? W[p] : INFINITY;
+        };
+        load_block(0u);
         for (uint32_t cb = 0; cb < total; cb += 32) {
-            const uint32_t g = cb + row_l;
-            const bool valid = g < total;
-            uint32_t sp = cand_pos(valid ? g : 0u, nrw, rstart, rexcl);
-            sp = valid ? sp : fpos;
+            const uint32_t sp = sp_n;
+            const float wc = wc_n;
             vec B[KS], Bl[KS];
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) {
-                B[ks] = load8<LP>(H + (uint64_t)sp * DP + ks * 16 + half * 8);
-                if (SPLIT) Bl[ks] = load8<LP>(Lo + (uint64_t)sp * DP + ks * 16 + half * 8);
+                B[ks] = Bn[ks];
+                if (SPLIT) Bl[ks] = Bln[ks];
             }
-            const float wc = valid ? W[sp] : INFINITY;
+            if (cb + 32 < total) load_block(cb + 32);
             // per row block: MFMAs, then a 16-bit survivor mask per lane
             uint32_t smv[RB];
 #pragma unroll
@@ -198,7 +214,13 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                 }
                 uint32_t sm = 0;
 #pragma unroll
-                for (int i = 0; i < 16; i++) sm |= (acc[i] >= wq[rb][i] + wc) ? (1u << i) : 0u;
+                for (int j = 0; j < 4; j++) {
+                    const float4 wv = *reinterpret_cast<const float4 *>(&s_wq[w][rb * 32 + j * 8 + half * 4]);
+                    sm |= (acc[4 * j + 0] >= wv.x + wc) ? (1u << (4 * j + 0)) : 0u;
+                    sm |= (acc[4 * j + 1] >= wv.y + wc) ? (1u << (4 * j + 1)) : 0u;
+                    sm |= (acc[4 * j + 2] >= wv.z + wc) ? (1u << (4 * j + 2)) : 0u;
+                    sm |= (acc[4 * j + 3] >= wv.w + wc) ? (1u << (4 * j + 3)) : 0u;
+                }
                 smv[rb] = sm;
                 if (ERR && sm) {
                     const double s2 = aux.sig[0] * aux.sig[0];
@@ -251,7 +273,7 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                 const uint32_t spc = shfl_u32(sp, col);  // lane col holds that candidate
                 if (e < stot) {
                     nrech++;
-                    const T dv = dist_mem8<T>(xs, ld, q0 + r, spc, a.d, mode, a.ilp);
+                    const T dv = dist_mem_batch<T, 16>(xs, ld, q0 + r, spc, a.d, mode, a.ilp);
                     if (dv <= thr) {
                         nhit++;
                         anyhit = true;

@@ -129,14 +129,13 @@ __device__ __forceinline__ T dist_mem(const T *__restrict__ xs, uint64_t ld, uin
     return acc;
 }
 
-// dist_mem with the coordinates of up to 8 dims loaded before their
-// arithmetic (16 loads in flight instead of a load round trip per dim); the
+// dist_mem with the coordinates of up to C dims loaded before their
+// arithmetic (2 C loads in flight instead of a load round trip per dim); the
 // floating-point operations and their order are dist_mem's.  Used by the
 // Gram filter's exact rechecks, where each lane checks a different pair.
-template <typename T>
-__device__ __forceinline__ T dist_mem8(const T *__restrict__ xs, uint64_t ld, uint32_t qpos, uint32_t cpos,
-                                       uint32_t d, int mode, uint32_t ilp) {
-    constexpr uint32_t C = 8;
+template <typename T, uint32_t C>
+__device__ __forceinline__ T dist_mem_batch(const T *__restrict__ xs, uint64_t ld, uint32_t qpos, uint32_t cpos,
+                                            uint32_t d, int mode, uint32_t ilp) {
     if (mode != ACC_ILP_NOSC) {
         // SEQ: an FMA per dim.  ILP_SC: an FMA for the first dim of each
         // group of ilp dims, a rounded product and an add for the others.
