@@ -53,6 +53,8 @@ struct gdsj_handle {
     bool use_gram = false;
     bool stamp = false;  // unicomp in force for this join
     DevBuf fcnt, rcur;   // unicomp: own-side counts, reverse-entry cursors
+    DevBuf maskW, hmCell, hmask;  // hit masks of the exact passes
+    bool use_hm = false;
     bool slotted = false;
     uint32_t novf = 0;
     gdsj_params p{};
@@ -149,6 +151,9 @@ int gdsj_destroy(gdsj_handle *h) {
     h->G.release();
     h->fcnt.release();
     h->rcur.release();
+    h->maskW.release();
+    h->hmCell.release();
+    h->hmask.release();
     DevBuf *b[] = {&h->counts,  &h->offsets,  &h->ids,     &h->data,    &h->scanTmp, &h->status,
                    &h->work,    &h->slotCap,  &h->slotOff, &h->slotUsed, &h->slotOvf, &h->slotIds,
                    &h->slotQ,   &h->scalars,  &h->estPred, &h->estScratch, &h->subTiles};
@@ -216,6 +221,11 @@ static JoinArgs make_args(gdsj_handle *h) {
         a.runs = h->G.runs.as<uint2>();
         a.nruns = h->G.nruns.as<uint32_t>();
         a.runStride = h->G.runStride;
+    }
+    if (h->use_hm) {
+        a.hmask = h->hmask.as<uint64_t>();
+        a.hmCell = h->hmCell.as<uint64_t>();
+        a.cellCand = h->G.cellCand.as<uint32_t>();
     }
     return a;
 }
@@ -397,11 +407,42 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
         GDSJ_TRY(h->gramStats.ensure(64));
         GDSJ_HIP(hipMemsetAsync(h->gramStats.p, 0, 24, s));
     }
+    // hit masks: exact passes over precomputed runs, plain kernels, no unicomp
+    // (auto: from 4 dims, where a pair's distance costs more than reading
+    // its mask bit back; at 2 dims the fill pass is faster recomputing)
+    h->use_hm = false;
+    const bool hm_ok = !h->use_gram && !h->stamp && h->p.estimator == GDSJ_EST_EXACT && !h->p.filter_stats;
+    if (h->p.hitmask == 1 && !hm_ok) {
+        set_error("hit masks need the exact passes without the Gram filter and without unicomp");
+        return GDSJ_EINVAL;
+    }
+    const bool want_hm = hm_ok && n > 0 && (h->p.hitmask == 1 || (h->p.hitmask == 0 && dims >= 4));
     {
         bool built = false;
         const bool prune = h->p.prune == 1 || (h->p.prune == 0 && h->G.k >= 4);
-        GDSJ_TRY(build_runs(h->G, s, prune, h->stamp, &built));
+        if (want_hm) GDSJ_TRY(h->maskW.ensure((size_t)h->G.ncell * 4 + 256));
+        GDSJ_TRY(build_runs(h->G, s, prune, h->stamp, &built, want_hm ? h->maskW.as<uint32_t>() : nullptr));
         h->G.haveRuns = built;
+    }
+    if (want_hm && h->G.haveRuns) {
+        // per-cell first mask word; the total decides whether they fit
+        const uint32_t nc = h->G.ncell;
+        GDSJ_TRY(h->hmCell.ensure(((size_t)nc + 1) * 8 + 256));
+        GDSJ_TRY(h->scanTmp.ensure(scan_tmp_bytes((uint64_t)nc + 1) + scan_tmp_bytes((uint64_t)n + 1) + 256));
+        GDSJ_TRY(exclusive_scan_u32_u64(h->maskW.as<uint32_t>(), h->hmCell.as<uint64_t>(), nc, true, h->scanTmp.p,
+                                        s));
+        uint64_t words = 0;
+        GDSJ_HIP(hipMemcpyAsync(&words, h->hmCell.as<uint64_t>() + nc, 8, hipMemcpyDeviceToHost, s));
+        GDSJ_HIP(hipStreamSynchronize(s));
+        const uint64_t limit = 8ull << 30;
+        if (words * 8 <= limit) {
+            GDSJ_TRY(h->hmask.ensure((size_t)words * 8 + 256));
+            h->use_hm = true;
+        } else if (h->p.hitmask == 1) {
+            set_error("hit masks need %llu bytes (limit %llu)", (unsigned long long)(words * 8),
+                      (unsigned long long)limit);
+            return GDSJ_EUNSUPPORTED;
+        }
     }
     uint64_t *offs = d_offsets;
     if (!offs) {

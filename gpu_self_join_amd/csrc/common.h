@@ -53,6 +53,13 @@ __device__ __forceinline__ uint32_t mbcnt_add(uint64_t m, uint32_t add) {
 
 __device__ __forceinline__ uint32_t popc64(uint64_t m) { return (uint32_t)__popcll(m); }
 
+// v_writelane_b32: lane `lane` (wave-uniform) of old replaced by the uniform
+// value v; the LLVM intrinsic has no clang builtin, so it is bound by name.
+extern "C" __device__ int gdsj_llvm_writelane_i32(int v, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+__device__ __forceinline__ uint32_t writelane_u32(uint32_t v, uint32_t lane, uint32_t old) {
+    return (uint32_t)gdsj_llvm_writelane_i32((int)v, (int)lane, (int)old);
+}
+
 // lanes 0 .. n-1 (all lanes for n >= 64)
 __device__ __forceinline__ uint64_t prefix_mask(uint32_t n) {
     return n >= (uint32_t)GDSJ_WAVE ? ~0ull : ((1ull << n) - 1ull);

@@ -504,6 +504,7 @@ int estimator_capacity_sampling(const Grid &G, const void *joinArgs, int mode, u
         a.ntiles = ns;
         a.counts = countsScratch;
         a.tileWork = nullptr;
+        a.hmask = nullptr;
         GDSJ_TRY(join_pass(G, a, mode, PASS_COUNT, s));
         k_sample_sum<<<ceil_div(ns, 256), 256, 0, s>>>(countsScratch, ns, stride, sum);
         uint64_t cnt = 0;

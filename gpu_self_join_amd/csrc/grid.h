@@ -18,6 +18,8 @@
 //                     (both replace the binary searches of kernel.cu:395-398)
 //   tiles     [nt]    uint4  {cell, q0, q1, 0}: up to 64 queries of one cell
 #pragma once
+#include <vector>
+
 #include "common.h"
 
 namespace gdsj {
@@ -53,6 +55,8 @@ struct Grid {
     // device buffers
     DevBuf xs, ids, cellKey, cellStart, cellCoord, hkeys, hvals, tiles;
     DevBuf runs, nruns, cellCand, cellHist;  // per-cell neighbor runs (build_runs)
+    DevBuf comboTab;                         // neighbor combinations (build_runs)
+    std::vector<uint4> hComboTab;            //   its host copy (source of the upload)
     uint64_t runStride = 0;
     bool haveRuns = false;
     DevBuf keyA, keyB, valB, tmp, aux, flag, tileKey, tileVal, cellAux, scalars;

@@ -319,7 +319,8 @@ __global__ void k_tiles_permute(const uint4 *__restrict__ in, const uint32_t *__
 // ---------------------------------------------------------------- host --
 void Grid::release() {
     DevBuf *b[] = {&xs, &ids, &cellKey, &cellStart, &cellCoord, &hkeys, &hvals, &tiles, &keyA, &keyB,
-                   &valB, &tmp, &aux, &flag, &tileKey, &tileVal, &cellAux, &scalars, &runs, &nruns, &cellCand, &cellHist};
+                   &valB, &tmp, &aux, &flag, &tileKey, &tileVal, &cellAux, &scalars, &runs, &nruns, &cellCand, &cellHist,
+                   &comboTab};
     for (DevBuf *p : b) p->release();
 }
 

@@ -63,7 +63,22 @@ struct JoinArgs {
     uint32_t *fcounts;
     uint32_t *rcur;
     uint32_t *status;  // [0] set when a fill write would exceed cap
+    // hit masks (exact passes over precomputed runs, no unicomp): the
+    // counting pass stores the hit ballot of every (query, 64-candidate
+    // chunk), the fill pass reads it back instead of evaluating distances.
+    // Masks of the query at sorted position p of cell c start at word
+    // hmCell[c] + (p - cellStart[c]) * stride(c), one word per chunk, with
+    // stride(c) = ceil(cellCand[c] / 64) + ceil(nruns[c] / 64) (a window of
+    // runs leaves at most one partial chunk).
+    uint64_t *hmask;            // nullable: no masks
+    const uint64_t *hmCell;
+    const uint32_t *cellCand;   // candidates per query of each cell
 };
+
+// mask words per query of a cell with cand candidates in nruns runs
+__host__ __device__ __forceinline__ uint32_t hmask_stride(uint32_t cand, uint32_t nruns) {
+    return (cand + 63u) / 64u + (nruns + 63u) / 64u;
+}
 
 // Launches one pass (JoinPass) of the join over a.ntiles tiles on stream s.
 int join_pass(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t s);
@@ -72,7 +87,9 @@ int join_pass(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t 
 // cellCand = candidates per query of the cell) when it fits the memory cap;
 // returns false (and leaves the lists unset) otherwise.
 // stamp: only the neighbor cells each cell owns (unicomp).
-int build_runs(Grid &G, hipStream_t s, bool prune, bool stamp, bool *built);
+// maskW (nullable): per cell, its queries' hit-mask words (saturated at
+// 2^32 - 1).
+int build_runs(Grid &G, hipStream_t s, bool prune, bool stamp, bool *built, uint32_t *maskW = nullptr);
 
 // out[0 .. *nout) = the tiles of list cut into sub-tiles of <= subq queries
 // (tileq: the tiles' maximum query count; out holds nlist * ceil(tileq/subq)).

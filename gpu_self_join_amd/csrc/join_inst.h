@@ -14,14 +14,19 @@ static int launch_join_t(const Grid &G, const JoinArgs &a, int pass, hipStream_t
         set_error("unicomp runs with the counting and fill passes only");
         return GDSJ_EINVAL;
     }
+    // hit masks with the exact passes over precomputed runs (no unicomp)
+    const bool hm = a.hmask != nullptr && !a.stamp && a.runs != nullptr;
+    const uint64_t *hmp = a.hmask;
     if (pass == PASS_FILL) {
-        if (a.stamp) k_join<JT, D, MODE, ILPV, PASS_FILL, true><<<blocks, 256, 0, s>>>(xs, a.ids, a);
-        else k_join<JT, D, MODE, ILPV, PASS_FILL><<<blocks, 256, 0, s>>>(xs, a.ids, a);
+        if (a.stamp) k_join<JT, D, MODE, ILPV, PASS_FILL, true><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
+        else if (hm) k_join<JT, D, MODE, ILPV, PASS_FILL, false, true><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
+        else k_join<JT, D, MODE, ILPV, PASS_FILL><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
     } else if (pass == PASS_SLOT) {
-        k_join<JT, D, MODE, ILPV, PASS_SLOT><<<blocks, 256, 0, s>>>(xs, a.ids, a);
+        k_join<JT, D, MODE, ILPV, PASS_SLOT><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
     } else {
-        if (a.stamp) k_join<JT, D, MODE, ILPV, PASS_COUNT, true><<<blocks, 256, 0, s>>>(xs, a.ids, a);
-        else k_join<JT, D, MODE, ILPV, PASS_COUNT><<<blocks, 256, 0, s>>>(xs, a.ids, a);
+        if (a.stamp) k_join<JT, D, MODE, ILPV, PASS_COUNT, true><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
+        else if (hm) k_join<JT, D, MODE, ILPV, PASS_COUNT, false, true><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
+        else k_join<JT, D, MODE, ILPV, PASS_COUNT><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
     }
     GDSJ_HIP(hipGetLastError());
     return GDSJ_OK;

@@ -19,10 +19,14 @@
 
 namespace gdsj {
 
+// candidate chunks held in VGPRs: about GDSJ_CMAX_WORDS coordinate words
+#ifndef GDSJ_CMAX_WORDS
+#define GDSJ_CMAX_WORDS 48
+#endif
 template <typename T, int D>
 struct CMaxFor {
     static constexpr int words = (int)sizeof(T) / 4;
-    static constexpr int raw = D > 0 ? 48 / (D * words) : 1;
+    static constexpr int raw = D > 0 ? GDSJ_CMAX_WORDS / (D * words) : 1;
     static constexpr int value = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
 };
 
@@ -199,13 +203,20 @@ __device__ __forceinline__ T dist_mem_batch(const T *__restrict__ xs, uint64_t l
 // capacity check); a single copy keeps the batch kernel's registers down.
 // BMAP: run-start bitmap for the candidate positions of precomputed runs
 // (else the binary search over the window's runs only).
+// HM (precomputed runs, no unicomp): hit masks (JoinArgs::hmask).  The
+// counting pass stores each query's chunk ballots; the fill pass takes its
+// hits from them and loads no coordinates.  Both passes cut the candidates
+// into the same windows and chunk batches, so chunk numbers agree.
 template <typename T, int D, int MODE, int ILPV, int PASS, int CM = 0, bool STAMP = false, int QRW = 16,
-          bool UNSW = true, bool BMAP = true>
+          bool UNSW = true, bool BMAP = true, bool HM = false>
 __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
-                                          const JoinArgs &a, const uint32_t tile) {
+                                          const JoinArgs &a, const uint32_t tile,
+                                          const uint64_t *__restrict__ hmr = nullptr) {
     constexpr int CMAX = CM > 0 ? CM : CMaxFor<T, D>::value;
     constexpr bool NEED_IDS = PASS != PASS_COUNT || STAMP;
     static_assert(!STAMP || PASS != PASS_SLOT, "unicomp runs with the counting and fill passes");
+    static_assert(!HM || (!STAMP && PASS != PASS_SLOT), "hit masks: exact passes without unicomp");
+    constexpr bool MFILL = HM && PASS == PASS_FILL;  // hits read from the masks
     const uint32_t lane = lane_id();
     const uint64_t ld = a.ld;
     const T thr = (T)a.thr;
@@ -247,7 +258,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
     }
     // low dims: lane l holds query l's coordinates; the query loop reads them
     // with readlane (no memory access per query)
-    constexpr bool QREG = D > 0 && D * (int)(sizeof(T) / 4) <= QRW;
+    constexpr bool QREG = !MFILL && D > 0 && D * (int)(sizeof(T) / 4) <= QRW;
     T qv[QREG ? D : 1];
     if (QREG) {
 #pragma unroll
@@ -265,6 +276,13 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
         sbase = uniform64(a.slotOff[tile]);
     }
     bool overflow = false;
+    // hit masks: word of (query qi, chunk j) = hbase + qi * hstride + j
+    uint64_t hbase = 0;
+    uint32_t hstride = 0, gch = 0;
+    if (HM) {
+        hstride = hmask_stride(uniform(a.cellCand[cell]), nwin);
+        hbase = uniform64(a.hmCell[cell]) + (uint64_t)(q0 - uniform(a.cellStart[cell])) * hstride;
+    }
 
     for (uint32_t cw = 0; cw < nwin; cw += GDSJ_WAVE) {
         const uint32_t nrw = uniform(nwin - cw < (uint32_t)GDSJ_WAVE ? nwin - cw : (uint32_t)GDSJ_WAVE);
@@ -346,7 +364,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                 sp = valid[ch] ? sp : fpos;
                 cpos[ch] = sp;
                 other[ch] = STAMP && valid[ch] && (sp < own0 || sp >= own1);
-                if (D > 0 && ch < (int)nch) {
+                if (!MFILL && D > 0 && ch < (int)nch) {
                     // lanes past the end read the NaN padding behind the last
                     // point (position ld - 1), which fails every compare
                     const uint64_t xp = valid[ch] ? (uint64_t)sp : ld - 1;
@@ -367,6 +385,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
 #undef GDSJ_RY
             }
             if (PASS == PASS_FILL) cur_l += inc_l;
+            if (HM) gch += nch;
             if (STAMP) {
                 // the reverse direction of this chunk batch's hits
 #pragma unroll
@@ -409,14 +428,15 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
     if (PASS == PASS_FILL && overflow) atomicOr(a.status, 1u);
 }
 
-template <typename T, int D, int MODE, int ILPV, int PASS, bool STAMP = false>
+// hm: the hit masks (HM; a.hmask), read-only here in the fill pass
+template <typename T, int D, int MODE, int ILPV, int PASS, bool STAMP = false, bool HM = false>
 __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
-                                              const JoinArgs a) {
+                                              const JoinArgs a, const uint64_t *__restrict__ hm) {
     const uint32_t lblock = a.xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t tslot = uniform((uint32_t)(lblock * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
     if (tslot >= a.ntiles) return;
     const uint32_t tile = a.tileList ? uniform(a.tileList[tslot]) : tslot;
-    join_tile<T, D, MODE, ILPV, PASS, 0, STAMP>(xs, ids, a, tile);
+    join_tile<T, D, MODE, ILPV, PASS, 0, STAMP, 16, true, true, HM>(xs, ids, a, tile, hm);
 }
 
 }  // namespace gdsj

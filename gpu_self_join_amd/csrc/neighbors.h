@@ -37,13 +37,19 @@ __device__ __forceinline__ uint32_t cand_pos(uint32_t g, uint32_t nrw, uint32_t 
 // evaluated from one side only: a row of dims 1..k-1 whose highest nonzero
 // offset is in dim j is kept iff c_j is odd; the cell's own row keeps the
 // dim-0 neighbors iff c_0 is odd (the own cell always).
-__device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK], const double *glo,
-                                           const double *ghi, const GridArgs &ga,
-                                           const uint32_t *__restrict__ cellStart,
-                                           const uint64_t *__restrict__ cellKey,
-                                           const uint64_t *__restrict__ hkeys,
-                                           const uint32_t *__restrict__ hvals, bool stamp = false) {
-    uint64_t base = 0;
+// The row of combination t: its linear id without dim 0 (base) and the
+// dim-0 range [x0, x1] left after the clamps and the pruning; live is false
+// when the whole row is skipped.
+struct ComboRow {
+    uint64_t base;
+    uint32_t x0, x1;
+    bool live;
+};
+
+__device__ __forceinline__ ComboRow combo_decode(uint32_t t, const uint32_t (&cc)[MAXK], const double *glo,
+                                                 const double *ghi, const GridArgs &ga, bool stamp) {
+    ComboRow w;
+    w.base = 0;
     bool ok = true;
     double gap = 0.0;
     uint32_t tt = t;
@@ -54,17 +60,29 @@ __device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK
         const int64_t cj = (int64_t)cc[j] + (int64_t)dlt - 1;
         ok &= (cj >= 0) && (cj < (int64_t)ga.nCells[j]);
         gap += dlt == 0 ? glo[j] : (dlt == 2 ? ghi[j] : 0.0);
-        base += (uint64_t)cj * ga.mult[j];
+        w.base += (uint64_t)cj * ga.mult[j];
         if (dlt != 1) top = (int)j;
     }
+    w.live = ok && !(gap > ga.prune_lim) && !(stamp && top >= 0 && !(cc[top] & 1u));
+    w.x0 = cc[0] == 0 ? 0u : cc[0] - 1u;
+    w.x1 = cc[0] + 1u < ga.nCells[0] ? cc[0] + 1u : ga.nCells[0] - 1u;
+    if (w.x0 < cc[0] && gap + glo[0] > ga.prune_lim) w.x0 = cc[0];
+    if (w.x1 > cc[0] && gap + ghi[0] > ga.prune_lim) w.x1 = cc[0];
+    if (stamp && top < 0 && !(cc[0] & 1u)) w.x0 = w.x1 = cc[0];
+    return w;
+}
+
+__device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK], const double *glo,
+                                           const double *ghi, const GridArgs &ga,
+                                           const uint32_t *__restrict__ cellStart,
+                                           const uint64_t *__restrict__ cellKey,
+                                           const uint64_t *__restrict__ hkeys,
+                                           const uint32_t *__restrict__ hvals, bool stamp = false) {
+    const ComboRow w = combo_decode(t, cc, glo, ghi, ga, stamp);
     uint2 run = make_uint2(0u, 0u);
-    if (!ok || gap > ga.prune_lim) return run;
-    if (stamp && top >= 0 && !(cc[top] & 1u)) return run;
-    uint32_t x0 = cc[0] == 0 ? 0u : cc[0] - 1u;
-    uint32_t x1 = cc[0] + 1u < ga.nCells[0] ? cc[0] + 1u : ga.nCells[0] - 1u;
-    if (x0 < cc[0] && gap + glo[0] > ga.prune_lim) x0 = cc[0];
-    if (x1 > cc[0] && gap + ghi[0] > ga.prune_lim) x1 = cc[0];
-    if (stamp && top < 0 && !(cc[0] & 1u)) x0 = x1 = cc[0];
+    if (!w.live) return run;
+    const uint64_t base = w.base;
+    const uint32_t x0 = w.x0, x1 = w.x1;
     if (ga.hbits) {
         // one probe for the row, then the cells x0..x1 of it by key: the
         // row's first four keys are read together (sorted, so the number
@@ -96,6 +114,112 @@ __device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK
     }
     if (first >= 0) run = make_uint2(cellStart[first], cellStart[last + 1]);
     return run;
+}
+
+// The runs of R decoded rows w[0..R) (combo_run's lookup half): each
+// lookup step (row probe, row keys, cell starts) issues its loads for all R
+// before the first of them is used, so a wave waits for R dependent chains
+// together instead of one after another.
+template <int R>
+__device__ __forceinline__ void combo_lookup(const ComboRow (&w)[R], const GridArgs &ga,
+                                             const uint32_t *__restrict__ cellStart,
+                                             const uint64_t *__restrict__ cellKey,
+                                             const uint64_t *__restrict__ hkeys,
+                                             const uint32_t *__restrict__ hvals, uint2 (&run)[R]) {
+#pragma unroll
+    for (int r = 0; r < R; r++) run[r] = make_uint2(0u, 0u);
+    int32_t first[R], last[R];  // cell range of the run, -1: none
+    if (ga.hbits) {
+        // first probe of every row (a table at most half full mostly
+        // answers there); the rare longer probe sequences continue below
+        const uint32_t mask = (1u << ga.hbits) - 1u;
+        uint32_t h[R];
+        uint64_t key[R];
+        uint32_t val[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            h[r] = cell_hash(w[r].base, ga.hbits);
+            key[r] = 0;
+            val[r] = 0;
+            if (w[r].live) {
+                const ulonglong2 e = reinterpret_cast<const ulonglong2 *>(hkeys)[h[r]];
+                key[r] = e.x;
+                val[r] = (uint32_t)e.y;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            first[r] = -1;
+            if (!w[r].live || key[r] == 0) continue;
+            if (key[r] == w[r].base + 1) {
+                first[r] = (int32_t)val[r];
+                continue;
+            }
+            uint32_t hh = (h[r] + 1) & mask;
+            for (uint32_t probe = 1; probe <= mask; probe++) {
+                const ulonglong2 e = reinterpret_cast<const ulonglong2 *>(hkeys)[hh];
+                if (e.x == w[r].base + 1) {
+                    first[r] = (int32_t)e.y;
+                    break;
+                }
+                if (e.x == 0) break;
+                hh = (hh + 1) & mask;
+            }
+        }
+        // the row's first four keys, for all rows
+        uint64_t kk[R][4];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (uint32_t i = 0; i < 4; i++) {
+                kk[r][i] = ~0ull;
+                if (first[r] >= 0 && (uint32_t)first[r] + i < ga.ncell) kk[r][i] = cellKey[(uint32_t)first[r] + i];
+            }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (first[r] < 0) continue;
+            const uint32_t rr = (uint32_t)first[r];
+            const uint64_t w0 = w[r].base + w[r].x0, w1 = w[r].base + w[r].x1 + 1;
+            uint32_t nf = 0, ne = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < 4; i++) {
+                nf += kk[r][i] < w0 ? 1u : 0u;
+                ne += kk[r][i] < w1 ? 1u : 0u;
+            }
+            uint32_t f = rr + nf, e = rr + ne;
+            if (nf == 4) f = key_lower_bound(cellKey, rr + 4, ga.ncell, w0);
+            if (ne == 4) e = key_lower_bound(cellKey, f, ga.ncell, w1);
+            // as a cell range [first, last]
+            first[r] = f < e ? (int32_t)f : -1;
+            last[r] = (int32_t)e - 1;
+        }
+    } else {
+        // direct table: the up to three cells of every row
+        int32_t ci[R][3];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (uint32_t x = 0; x < 3; x++) {
+                ci[r][x] = -1;
+                if (w[r].live && w[r].x0 + x <= w[r].x1) ci[r][x] = (int32_t)hvals[w[r].base + w[r].x0 + x];
+            }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            first[r] = -1;
+            last[r] = -1;
+#pragma unroll
+            for (uint32_t x = 0; x < 3; x++) {
+                if (ci[r][x] >= 0) {
+                    if (first[r] < 0) first[r] = ci[r][x];
+                    last[r] = ci[r][x];
+                }
+            }
+        }
+    }
+    // the cell starts of every run
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        if (first[r] >= 0) run[r] = make_uint2(cellStart[first[r]], cellStart[last[r] + 1]);
 }
 
 // Cell coordinates of a cell in the k indexed dims, all loads issued
@@ -131,23 +255,24 @@ __device__ __forceinline__ void tile_gaps(const T *__restrict__ xs, uint64_t ld,
             const uint32_t jj = (uint32_t)j < ga.k ? (uint32_t)j : 0u;
             raw[j] = xs[(uint64_t)jj * ld + q0 + pl];
         }
-        double v[MAXK];
-#pragma unroll
-        for (int j = 0; j < MAXK; j++) v[j] = (double)raw[j];
+        // min / max in DTYPE (exact, so the double conversion after it gives
+        // the same bounds) over the lanes [0, 2^steps) that cover the n
+        // points; lanes past n hold the neutral values
+        const uint32_t steps = n <= 1u ? 0u : 32u - (uint32_t)__builtin_clz(n - 1u);
 #pragma unroll
         for (int j = 0; j < MAXK; j++) {
             if ((uint32_t)j >= ga.k) break;
-            double mn = lane < n ? v[j] : 1e300, mx = lane < n ? v[j] : -1e300;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
+            T mn = lane < n ? raw[j] : (T)__builtin_inf(), mx = lane < n ? raw[j] : -(T)__builtin_inf();
+            for (uint32_t st = 0; st < steps; st++) {
+                const int off = 1 << st;
                 mn = fmin(mn, __shfl_xor(mn, off, GDSJ_WAVE));
                 mx = fmax(mx, __shfl_xor(mx, off, GDSJ_WAVE));
             }
             const double lo_face = ga.minArr[j] + (double)cc[j] * ga.eps;
             const double hi_face = lo_face + ga.eps;
             const double tol = 8.0 * ga.ulp * (fabs(ga.minArr[j]) + fabs(hi_face) + 2.0 * ga.eps);
-            const double gl = fmax(0.0, mn - lo_face - tol);
-            const double gh = fmax(0.0, hi_face - tol - mx);
+            const double gl = fmax(0.0, (double)mn - lo_face - tol);
+            const double gh = fmax(0.0, hi_face - tol - (double)mx);
             if (lane == 0) {
                 glo[j] = gl * gl;
                 ghi[j] = gh * gh;

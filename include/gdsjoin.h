@@ -74,7 +74,11 @@ typedef struct gdsj_params {
                                join, 2 never */
     uint32_t filter_stats;  /* 1: the Gram filter also measures its error on the rechecked
                                pairs (gdsj_stats.filter_max_err; slower) */
-    uint32_t reserved;
+    uint32_t hitmask;       /* exact passes without the Gram filter: the counting pass keeps
+                               one bit per candidate pair and the fill pass reads them back
+                               instead of recomputing distances: 0 auto (on from 4 dims
+                               when they take at most 8 GiB), 1 on (error if they do not
+                               fit), 2 off (speed only) */
 } gdsj_params;
 
 /* Timing and diagnostics of the last join on a handle. */

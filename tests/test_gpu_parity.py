@@ -145,6 +145,44 @@ def test_many_runs_window(sj):
     _check(sj, x, 0.9, k=6, ilp=8, sc=1, reorder=1)
 
 
+HITMASK_CASES = [
+    # (name, n, d, k, dtype, ilp, sc, reorder, data scale, eps)
+    ("uniform2d", 20000, 2, 2, np.float32, 0, 0, 0, 100.0, 1.0),
+    ("dense_cell_tiles", 5000, 2, 2, np.float32, 0, 0, 0, 3.0, 1.0),
+    ("k6_many_runs", 20000, 6, 6, np.float32, 8, 1, 1, 4.0, 0.9),
+    ("k3_of_5_runtime_dims", 8000, 5, 3, np.float32, 0, 0, 0, 10.0, 1.2),
+    ("d10_generic", 6000, 10, 6, np.float32, 0, 0, 1, 3.0, 1.1),
+    ("f64_d4_nosc", 8000, 4, 4, np.float64, 8, 0, 0, 8.0, 1.0),
+]
+
+
+@pytest.mark.parametrize("case", HITMASK_CASES, ids=lambda c: c[0])
+def test_hitmask_fill_equals_recomputing_fill(sj, case):
+    """The fill pass that reads the counting pass's hit masks writes the
+    same table, byte for byte (same row order), as the fill pass that
+    evaluates every distance again, and both equal the oracle."""
+    name, n, d, k, dt, ilp, sc, reorder, scale, eps = case
+    rng = np.random.default_rng(n + d)
+    x = (rng.random((n, d)) * scale).astype(dt)
+    eps = float(np.asarray(eps, dtype=dt))
+    kw = dict(k=k, ilp=ilp, shortcircuit=sc, reorder=reorder, stamp=0, estimator=0)
+    c1, ids1, r1 = _gpu_join(sj, x, eps, hitmask=1, **kw)
+    c2, ids2, r2 = _gpu_join(sj, x, eps, hitmask=2, **kw)
+    print("hitmask %s total=%d" % (name, r1.total), flush=True)
+    assert np.array_equal(c1, c2)
+    assert np.array_equal(ids1, ids2)
+    ref = O.join(x, eps, k=k, ilp=ilp, shortcircuit=sc, reorder=reorder)
+    assert_csr_equal(c1, ids1, ref["counts"], ref["ids"])
+
+
+def test_hitmask_rejected_where_unsupported(sj):
+    x = (np.random.default_rng(5).random((2000, 2)) * 10).astype(np.float32)
+    with pytest.raises(_lib.GdsjError):
+        _gpu_join(sj, x, 1.0, k=2, stamp=1, hitmask=1)
+    with pytest.raises(_lib.GdsjError):
+        _gpu_join(sj, x, 1.0, k=2, estimator=1, hitmask=1)
+
+
 def test_edge_single_point(sj):
     x = np.array([[1.0, 2.0, 3.0]], np.float32)
     c, ids, r = _gpu_join(sj, x, 0.5, k=3)
