@@ -110,6 +110,22 @@ def cpu_reference_paths(x, eps, budget_s=8.0, seed=321):
 
 EST = {"exact": 0, "learned": 1, "sample": 2, "plain": 0, "nostamp": 0}
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def measured_traffic(workload, kernel_label):
+    """HBM bytes per launch of a kernel (read + written), from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain
+    copy of known size); None when that workload and kernel were not
+    measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tab = json.load(f)
+        e = tab["workloads"][workload][kernel_label]
+        return int(e["read_bytes"] + e["write_bytes"]), e
+    except (OSError, KeyError, ValueError):
+        return None, None
+
 
 def copy_probe(nbytes, reps=5):
     """A plain device copy of nbytes (the calibration kernel for the HBM
@@ -369,9 +385,13 @@ def main():
         kname["compact"] = "k_compact"
     dom = max(kms, key=kms.get)
     achieved = kb[dom] / (kms[dom] * 1e-3) / 1e9
+    traffic, tsrc = measured_traffic(name, kname[dom])
     roof = {"bound": "hbm", "kernel": kname[dom], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "alg_bytes_per_launch": kb[dom], "avg_launch_ms": round(kms[dom], 4)}
+    if tsrc:
+        roof["traffic_detail"] = {"read_bytes": tsrc["read_bytes"], "write_bytes": tsrc["write_bytes"],
+                                  "source": "profiles/hbm_traffic.json", "kernel": tsrc["kernel"]}
     if s["filter_rechecks"] == 0 and dom != "compact":
         # the plain pass evaluates every candidate pair: a subtract and an FMA
         # per dim (3 D flops), against the FP32 / FP64 vector peak
@@ -393,7 +413,8 @@ def main():
         tf = flops / (kms[dom] * 1e-3) / 1e12
         hbm = dict(roof)
         roof = {"bound": "mfma", "kernel": kname[dom].replace("k_join", "k_gram"), "achieved": round(tf, 3),
-                "peak": MFMA_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / MFMA_PEAK_TFLOPS, 5), "traffic": None,
+                "peak": MFMA_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / MFMA_PEAK_TFLOPS, 5),
+                "traffic": measured_traffic(name, kname[dom].replace("k_join", "k_gram"))[0],
                 "alg_flops_per_launch": flops, "avg_launch_ms": round(kms[dom], 4),
                 "hbm": {k: hbm[k] for k in ("achieved", "peak", "unit", "frac", "alg_bytes_per_launch")}}
     variants = {}

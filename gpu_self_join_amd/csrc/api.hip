@@ -17,6 +17,7 @@
 #include <mutex>
 #include <vector>
 
+#include "bruteforce.h"
 #include "estimator.h"
 #include "gram.h"
 #include "grid.h"
@@ -54,6 +55,7 @@ struct gdsj_handle {
     bool stamp = false;  // unicomp in force for this join
     DevBuf fcnt, rcur;   // unicomp: own-side counts, reverse-entry cursors
     DevBuf maskW, hmCell, hmask;  // hit masks of the exact passes
+    DevBuf bfXs;                  // all-pairs join: SoA copy of the points
     bool use_hm = false;
     bool slotted = false;
     uint32_t novf = 0;
@@ -152,6 +154,7 @@ int gdsj_destroy(gdsj_handle *h) {
     h->fcnt.release();
     h->rcur.release();
     h->maskW.release();
+    h->bfXs.release();
     h->hmCell.release();
     h->hmask.release();
     DevBuf *b[] = {&h->counts,  &h->offsets,  &h->ids,     &h->data,    &h->scanTmp, &h->status,
@@ -524,6 +527,90 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     h->st.ms_total = h->st.ms_grid + h->st.ms_estimate + h->st.ms_count + h->st.ms_scan + h->st.ms_compact +
                      h->st.ms_fill + h->st.ms_sort;
     return rc;
+}
+
+// All-pairs join (the reference's search mode 1, GPU.cu:1285-1480,
+// kernel.cu:486-515): counts, offsets and, when d_ids holds them, the rows.
+int gdsj_bruteforce_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t dims, int dtype,
+                           double epsilon, const gdsj_params *pin, uint32_t *d_counts, uint64_t *d_offsets,
+                           uint32_t *d_ids, uint64_t ids_cap, uint64_t *total, void *stream) {
+    if (!h || (!d_data && n) || (!d_counts && n) || !total) return GDSJ_EINVAL;
+    if (dtype != GDSJ_DTYPE_F32 && dtype != GDSJ_DTYPE_F64) {
+        set_error("dtype must be GDSJ_DTYPE_F32 or GDSJ_DTYPE_F64");
+        return GDSJ_EINVAL;
+    }
+    if (dims == 0 || dims > (uint32_t)MAXD) {
+        set_error("dims=%u must be in 1..%d", dims, MAXD);
+        return GDSJ_EINVAL;
+    }
+    if (!(epsilon > 0) || !std::isfinite(epsilon)) {
+        set_error("epsilon must be positive and finite");
+        return GDSJ_EINVAL;
+    }
+    gdsj_params p;
+    if (pin) p = *pin; else gdsj_params_default(&p);
+    if (p.ilp > 64) {
+        set_error("ILP=%u exceeds 64", p.ilp);
+        return GDSJ_EINVAL;
+    }
+    double thr;
+    if (dtype == GDSJ_DTYPE_F32) {
+        const float e = (float)epsilon;
+        if ((double)e != epsilon) {
+            set_error("epsilon %.17g is not a float value", epsilon);
+            return GDSJ_EINVAL;
+        }
+        thr = (double)gdsj_eps_threshold_f32(e);
+    } else {
+        thr = gdsj_eps_threshold_f64(epsilon);
+    }
+    const int mode = p.ilp == 0 ? ACC_SEQ : (p.shortcircuit ? ACC_ILP_SC : ACC_ILP_NOSC);
+    GDSJ_HIP(hipSetDevice(h->device));
+    h->have_counts = h->have_table = false;
+    hipStream_t s = stream ? (hipStream_t)stream : h->own;
+    const size_t es = dtype == GDSJ_DTYPE_F64 ? 8 : 4;
+    const uint64_t ld = ((uint64_t)n + 63) / 64 * 64 + 64;
+    GDSJ_TRY(h->bfXs.ensure((size_t)ld * dims * es + 256));
+    uint64_t *offs = d_offsets;
+    if (!offs) {
+        GDSJ_TRY(h->offsets.ensure(((size_t)n + 1) * 8));
+        offs = h->offsets.as<uint64_t>();
+    }
+    GDSJ_TRY(h->scanTmp.ensure(scan_tmp_bytes((uint64_t)n + 1) + 256));
+    GDSJ_TRY(bruteforce_soa(d_data, n, dims, dtype, ld, h->bfXs.p, s));
+    GDSJ_TRY(bruteforce_pass(h->bfXs.p, ld, n, dims, dtype, mode, p.ilp, thr, PASS_COUNT, d_counts, nullptr, nullptr,
+                             s));
+    GDSJ_TRY(exclusive_scan_u32_u64(d_counts, offs, n, true, h->scanTmp.p, s));
+    uint64_t tot = 0;
+    GDSJ_HIP(hipMemcpyAsync(&tot, offs + n, 8, hipMemcpyDeviceToHost, s));
+    GDSJ_HIP(hipStreamSynchronize(s));
+    *total = tot;
+    if (!d_ids) return GDSJ_OK;
+    if (tot > ids_cap) {
+        set_error("result has %llu pairs, output buffer holds %llu", (unsigned long long)tot,
+                  (unsigned long long)ids_cap);
+        return GDSJ_EOVERFLOW;
+    }
+    GDSJ_TRY(bruteforce_pass(h->bfXs.p, ld, n, dims, dtype, mode, p.ilp, thr, PASS_FILL, nullptr, offs, d_ids, s));
+    GDSJ_HIP(hipStreamSynchronize(s));
+    return GDSJ_OK;
+}
+
+// Host-buffer form (the CLI's mode 1): counts[n] to host and the total.
+int gdsj_bruteforce_host(gdsj_handle *h, const void *data, uint32_t n, uint32_t dims, int dtype, double epsilon,
+                         const gdsj_params *p, uint32_t *counts, uint64_t *total) {
+    if (!h || (!data && n) || (!counts && n) || !total) return GDSJ_EINVAL;
+    GDSJ_HIP(hipSetDevice(h->device));
+    const size_t bytes = (size_t)n * dims * (dtype == GDSJ_DTYPE_F64 ? 8 : 4);
+    GDSJ_TRY(h->data.ensure(bytes + 256));
+    GDSJ_TRY(h->counts.ensure((size_t)n * 4 + 256));
+    hipStream_t s = h->own;
+    if (bytes) GDSJ_HIP(hipMemcpyAsync(h->data.p, data, bytes, hipMemcpyHostToDevice, s));
+    GDSJ_TRY(gdsj_bruteforce_device(h, h->data.p, n, dims, dtype, epsilon, p, h->counts.as<uint32_t>(), nullptr,
+                                    nullptr, 0, total, s));
+    if (n) GDSJ_HIP(hipMemcpyAsync(counts, h->counts.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    GDSJ_HIP(hipStreamSynchronize(s));
+    return GDSJ_OK;
 }
 
 int gdsj_fill_device(gdsj_handle *h, uint32_t *d_ids, uint64_t ids_cap) {

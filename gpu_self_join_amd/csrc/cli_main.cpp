@@ -1,7 +1,9 @@
-// cli_main.cpp -- the command-line join: gdsjoin_main <file> <eps> <dim> 3
+// cli_main.cpp -- the command-line join: gdsjoin_main <file> <eps> <dim> 3|1
 //
 // Drop-in for the reference's `main <dataset> <epsilon> <dims> <mode>`
-// (main.cu:66-120, mode 3 at main.cu:178-331): imports the text dataset
+// (main.cu:66-120, mode 3 at main.cu:178-331, mode 1 at main.cu:150-175):
+// mode 3 is the grid join, mode 1 the all-pairs GPU join (it prints the
+// total as the reference's brute force does).  Mode 3 imports the text dataset
 // (gdsj_csv_load, import_dataset.cpp:49-95), runs the grid self-join on the
 // GPU through the C ABI, prints the time and the total, and appends one line
 // to gpu_stats.txt in the reference's format (main.cu:257-262).  The
@@ -10,7 +12,11 @@
 //   --dtype float|double (default float)  --k K (default min(dim, 6))
 //   --ilp I  --shortcircuit 0|1  --reorder 0|1  --stamp 0|1  --queryreorder 0|1
 //   --estimator exact|learned|sample  --filter plain|gram|auto  --device D
-//   --print   print every point's sorted neighbor list (PRINTNEIGHBORTABLE)
+//   --print   print every point's sorted neighbor list (PRINTNEIGHBORTABLE
+//             with STAMP, main.cu:299-322)
+//   --table-file    write DSSJ_out.txt (printNeighborTable main.cu:544-565)
+//   --outlier-scores  write DSSJ_outlier_scores.txt (printOutlierScores
+//             main.cu:568-626)
 // Errors exit with a non-zero status (the reference prints and returns 0).
 #include <stdint.h>
 #include <stdio.h>
@@ -28,11 +34,57 @@
 static int usage() {
     fprintf(stderr,
             "usage: gdsjoin_main <dataset file> <epsilon> <dims> <mode> [options]\n"
-            "  mode 3: GPU grid self-join (the only mode of this build)\n"
+            "  mode 3: GPU grid self-join; mode 1: all-pairs GPU join (total only)\n"
             "  options: --dtype float|double --k K --ilp I --shortcircuit 0|1 --reorder 0|1\n"
             "           --stamp 0|1 --queryreorder 0|1 --estimator exact|learned|sample\n"
-            "           --filter plain|gram|auto --device D --print\n");
+            "           --filter plain|gram|auto --device D --print --table-file --outlier-scores\n");
     return 2;
+}
+
+// DSSJ_out.txt as printNeighborTable writes it (main.cu:544-565): a header,
+// then one line per point id with its neighbor ids ascending, each followed
+// by ", ".  ids holds the CSR rows in point order.
+static void write_table(const std::vector<uint32_t> &counts, std::vector<uint32_t> &ids, uint64_t npts) {
+    const char *fname = "DSSJ_out.txt";
+    FILE *f = fopen(fname, "w");
+    if (!f) {
+        fprintf(stderr, "\nERROR: cannot write %s\n", fname);
+        return;
+    }
+    printf("\n\nOutputting neighbors to: %s\n", fname);
+    fprintf(f, "#data point (line is the point id), neighbor point ids\n");
+    uint64_t off = 0;
+    for (uint64_t i = 0; i < npts; i++) {
+        std::sort(ids.begin() + (long)off, ids.begin() + (long)(off + counts[i]));
+        for (uint32_t j = 0; j < counts[i]; j++) fprintf(f, "%u, ", ids[off + j]);
+        fprintf(f, "\n");
+        off += counts[i];
+    }
+    fclose(f);
+}
+
+// DSSJ_outlier_scores.txt as printOutlierScores writes it (main.cu:568-626):
+// per point id, its neighbor count and its rank when the points are ordered
+// by ascending count (0 = fewest neighbors).  The reference orders with
+// std::sort, which leaves the order of equal counts unspecified; here equal
+// counts keep ascending point ids (a stable order).
+static void write_outlier_scores(const std::vector<uint32_t> &counts, uint64_t npts) {
+    const char *fname = "DSSJ_outlier_scores.txt";
+    std::vector<uint32_t> order(npts);
+    for (uint64_t i = 0; i < npts; i++) order[i] = (uint32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return counts[a] < counts[b]; });
+    std::vector<uint32_t> rank(npts);
+    for (uint64_t r = 0; r < npts; r++) rank[order[r]] = (uint32_t)r;
+    FILE *f = fopen(fname, "w");
+    if (!f) {
+        fprintf(stderr, "\nERROR: cannot write %s\n", fname);
+        return;
+    }
+    printf("\nOutputting outlier scores to: %s\n", fname);
+    fprintf(f, "#data point (line is the point id), col0: Number of neighbors the point has within epsilon, ");
+    fprintf(f, "col1: outlier ranking for col0\n");
+    for (uint64_t i = 0; i < npts; i++) fprintf(f, "%u, %u\n", counts[i], rank[i]);
+    fclose(f);
 }
 
 static int fail(const char *what, int rc) {
@@ -50,7 +102,7 @@ int main(int argc, char **argv) {
     const double eps_arg = atof(argv[2]);  // main.cu:95: DTYPE epsilon = atof(...)
     const int ndim = atoi(argv[3]);
     const int mode = atoi(argv[4]);
-    int dtype = GDSJ_DTYPE_F32, device = 0, print = 0;
+    int dtype = GDSJ_DTYPE_F32, device = 0, print = 0, table_file = 0, outlier = 0;
     gdsj_params p;
     gdsj_params_default(&p);
     for (int i = 5; i < argc; i++) {
@@ -58,6 +110,14 @@ int main(int argc, char **argv) {
         const char *v = i + 1 < argc ? argv[i + 1] : nullptr;
         if (!strcmp(a, "--print")) {
             print = 1;
+            continue;
+        }
+        if (!strcmp(a, "--table-file")) {
+            table_file = 1;
+            continue;
+        }
+        if (!strcmp(a, "--outlier-scores")) {
+            outlier = 1;
             continue;
         }
         if (!v) return usage();
@@ -91,8 +151,9 @@ int main(int argc, char **argv) {
         fprintf(stderr, "\nERROR: the number of dimensions must be positive\n");
         return 1;
     }
-    if (mode != 3) {
-        fprintf(stderr, "\nERROR: search mode %d is not part of this build (mode 3: GPU grid join)\n", mode);
+    if (mode != 3 && mode != 1) {
+        fprintf(stderr, "\nERROR: search mode %d is not part of this build (3: GPU grid join, 1: all-pairs "
+                        "GPU join)\n", mode);
         return 2;
     }
     // DTYPE epsilon = atof(...): the double value rounded once to DTYPE
@@ -132,6 +193,27 @@ int main(int argc, char **argv) {
     }
     std::vector<uint32_t> counts(npts ? npts : 1);
     uint64_t total = 0;
+    if (mode == 1) {
+        // main.cu:150-175: the all-pairs join in the reference kernel's
+        // sequential accumulation (kernel.cu:500-503)
+        printf("\nBrute force GPU (NO BATCHING):");
+        gdsj_params bp = p;
+        bp.ilp = 0;
+        const auto b0 = std::chrono::steady_clock::now();
+        rc = gdsj_bruteforce_host(h, data.data(), (uint32_t)npts, (uint32_t)ndim, dtype, eps, &bp, counts.data(),
+                                  &total);
+        const auto b1 = std::chrono::steady_clock::now();
+        if (rc) {
+            gdsj_destroy(h);
+            return fail("brute force", rc);
+        }
+        printf("\nBRUTE FORCE Time on GPU: %f", std::chrono::duration<double>(b1 - b0).count());
+        printf("\nTotal neighbours in table: %llu", (unsigned long long)total);
+        printf("\n*********************************\n");
+        if (outlier) write_outlier_scores(counts, npts);
+        gdsj_destroy(h);
+        return 0;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     rc = gdsj_join_host(h, data.data(), (uint32_t)npts, (uint32_t)ndim, dtype, eps, &p, counts.data(), &total);
     const auto t1 = std::chrono::steady_clock::now();
@@ -156,6 +238,16 @@ int main(int argc, char **argv) {
           << std::endl;
     stats.close();
 
+    if (outlier) write_outlier_scores(counts, npts);
+    if (print || table_file) {
+        std::vector<uint32_t> ids(total ? total : 1);
+        rc = gdsj_result_copy(h, ids.data(), total);
+        if (rc) {
+            gdsj_destroy(h);
+            return fail("result copy", rc);
+        }
+        if (table_file) write_table(counts, ids, npts);
+    }
     if (print) {
         std::vector<uint32_t> ids(total ? total : 1);
         rc = gdsj_result_copy(h, ids.data(), total);

@@ -239,7 +239,24 @@ __device__ __forceinline__ void load_cell_coords(const uint32_t *__restrict__ ce
 // positions are widened by the classification rounding of (x - min) / eps
 // (a few units of DTYPE roundoff), so the gaps never overstate the distance
 // to any point of the neighbor cell.  Written by lane 0 (wave-uniform).
+// FEW (the run-list kernel: a wave per cell, typically a few points): the
+// min / max in DTYPE over ceil(log2 n) butterfly steps instead of six steps
+// in double; it needs more registers, which the join kernels cannot spare.
 template <typename T>
+__device__ __forceinline__ void store_gaps(double mn, double mx, uint32_t j, const uint32_t (&cc)[MAXK],
+                                           const GridArgs &ga, double *glo, double *ghi) {
+    const double lo_face = ga.minArr[j] + (double)cc[j] * ga.eps;
+    const double hi_face = lo_face + ga.eps;
+    const double tol = 8.0 * ga.ulp * (fabs(ga.minArr[j]) + fabs(hi_face) + 2.0 * ga.eps);
+    const double gl = fmax(0.0, mn - lo_face - tol);
+    const double gh = fmax(0.0, hi_face - tol - mx);
+    if (lane_id() == 0) {
+        glo[j] = gl * gl;
+        ghi[j] = gh * gh;
+    }
+}
+
+template <typename T, bool FEW = false>
 __device__ __forceinline__ void tile_gaps(const T *__restrict__ xs, uint64_t ld, uint32_t q0, uint32_t n,
                                           const uint32_t (&cc)[MAXK], const GridArgs &ga, double *glo,
                                           double *ghi) {
@@ -255,27 +272,36 @@ __device__ __forceinline__ void tile_gaps(const T *__restrict__ xs, uint64_t ld,
             const uint32_t jj = (uint32_t)j < ga.k ? (uint32_t)j : 0u;
             raw[j] = xs[(uint64_t)jj * ld + q0 + pl];
         }
-        // min / max in DTYPE (exact, so the double conversion after it gives
-        // the same bounds) over the lanes [0, 2^steps) that cover the n
-        // points; lanes past n hold the neutral values
-        const uint32_t steps = n <= 1u ? 0u : 32u - (uint32_t)__builtin_clz(n - 1u);
+        if constexpr (FEW) {
+            // exact in DTYPE, so the double conversion after it gives the
+            // same bounds; lanes [0, 2^steps) cover the n points, lanes past
+            // n hold the neutral values
+            const uint32_t steps = n <= 1u ? 0u : 32u - (uint32_t)__builtin_clz(n - 1u);
 #pragma unroll
-        for (int j = 0; j < MAXK; j++) {
-            if ((uint32_t)j >= ga.k) break;
-            T mn = lane < n ? raw[j] : (T)__builtin_inf(), mx = lane < n ? raw[j] : -(T)__builtin_inf();
-            for (uint32_t st = 0; st < steps; st++) {
-                const int off = 1 << st;
-                mn = fmin(mn, __shfl_xor(mn, off, GDSJ_WAVE));
-                mx = fmax(mx, __shfl_xor(mx, off, GDSJ_WAVE));
+            for (int j = 0; j < MAXK; j++) {
+                if ((uint32_t)j >= ga.k) break;
+                T mn = lane < n ? raw[j] : (T)__builtin_inf(), mx = lane < n ? raw[j] : -(T)__builtin_inf();
+                for (uint32_t st = 0; st < steps; st++) {
+                    const int off = 1 << st;
+                    mn = fmin(mn, __shfl_xor(mn, off, GDSJ_WAVE));
+                    mx = fmax(mx, __shfl_xor(mx, off, GDSJ_WAVE));
+                }
+                store_gaps<T>((double)mn, (double)mx, (uint32_t)j, cc, ga, glo, ghi);
             }
-            const double lo_face = ga.minArr[j] + (double)cc[j] * ga.eps;
-            const double hi_face = lo_face + ga.eps;
-            const double tol = 8.0 * ga.ulp * (fabs(ga.minArr[j]) + fabs(hi_face) + 2.0 * ga.eps);
-            const double gl = fmax(0.0, (double)mn - lo_face - tol);
-            const double gh = fmax(0.0, hi_face - tol - (double)mx);
-            if (lane == 0) {
-                glo[j] = gl * gl;
-                ghi[j] = gh * gh;
+        } else {
+            double v[MAXK];
+#pragma unroll
+            for (int j = 0; j < MAXK; j++) v[j] = (double)raw[j];
+#pragma unroll
+            for (int j = 0; j < MAXK; j++) {
+                if ((uint32_t)j >= ga.k) break;
+                double mn = lane < n ? v[j] : 1e300, mx = lane < n ? v[j] : -1e300;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    mn = fmin(mn, __shfl_xor(mn, off, GDSJ_WAVE));
+                    mx = fmax(mx, __shfl_xor(mx, off, GDSJ_WAVE));
+                }
+                store_gaps<T>(mn, mx, (uint32_t)j, cc, ga, glo, ghi);
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -295,15 +321,7 @@ __device__ __forceinline__ void tile_gaps(const T *__restrict__ xs, uint64_t ld,
             mn = fmin(mn, __shfl_xor(mn, off, GDSJ_WAVE));
             mx = fmax(mx, __shfl_xor(mx, off, GDSJ_WAVE));
         }
-        const double lo_face = ga.minArr[j] + (double)cc[j] * ga.eps;
-        const double hi_face = lo_face + ga.eps;
-        const double tol = 8.0 * ga.ulp * (fabs(ga.minArr[j]) + fabs(hi_face) + 2.0 * ga.eps);
-        const double gl = fmax(0.0, mn - lo_face - tol);
-        const double gh = fmax(0.0, hi_face - tol - mx);
-        if (lane == 0) {
-            glo[j] = gl * gl;
-            ghi[j] = gh * gh;
-        }
+        store_gaps<T>(mn, mx, j, cc, ga, glo, ghi);
     }
     __builtin_amdgcn_wave_barrier();
 }

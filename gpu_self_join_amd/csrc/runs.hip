@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R > 1 ? 4 :
     load_cell_coords(cellCoord, ga.ncell, cell, ga.k, cc);
     const uint32_t s0 = uniform(cellStart[cell]), s1 = uniform(cellStart[cell + 1]);
     if (prune) {
-        tile_gaps<T>(xs, ld, s0, s1 - s0, cc, ga, glo, ghi);
+        tile_gaps<T, true>(xs, ld, s0, s1 - s0, cc, ga, glo, ghi);
     } else {
         if (lane < MAXK) {
             glo[lane] = 0.0;

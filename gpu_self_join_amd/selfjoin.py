@@ -98,6 +98,44 @@ class SelfJoin:
             _lib.check(rc)
         return JoinResult(counts=counts, offsets=offsets, ids=out_ids[:tot], total=tot, stats=self.stats())
 
+    def bruteforce(self, points, eps, params=None, stream=None, **kw):
+        """All-pairs join of `points` (the reference's search mode 1,
+        GPU.cu:1285-1480): every ordered pair tested with the join's
+        arithmetic for params.ilp / shortcircuit, dims in input order.
+        Returns a JoinResult whose rows are ascending (stats: none)."""
+        torch = _torch()
+        if points.dim() != 2 or not points.is_cuda:
+            raise ValueError("points must be a 2-D cuda tensor")
+        points = points.contiguous()
+        n, d = points.shape
+        if points.dtype == torch.float32:
+            dt = _lib.DTYPE_F32
+            eps = float(np.float32(eps))
+        elif points.dtype == torch.float64:
+            dt = _lib.DTYPE_F64
+            eps = float(eps)
+        else:
+            raise TypeError("points must be float32 or float64")
+        if params is None:
+            params = _lib.default_params(**kw)
+        elif kw:
+            raise TypeError("pass either params or keyword fields")
+        dev = points.device
+        counts = torch.empty(n, dtype=torch.int32, device=dev)
+        offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        total = C.c_uint64(0)
+        if stream is None:
+            stream = torch.cuda.current_stream(dev).cuda_stream
+        _lib.check(self.L.gdsj_bruteforce_device(self.h, points.data_ptr(), n, d, dt, eps, C.byref(params),
+                                                 counts.data_ptr(), offsets.data_ptr(), None, 0, C.byref(total),
+                                                 stream))
+        tot = int(total.value)
+        ids = torch.empty(max(tot, 1), dtype=torch.int32, device=dev)
+        _lib.check(self.L.gdsj_bruteforce_device(self.h, points.data_ptr(), n, d, dt, eps, C.byref(params),
+                                                 counts.data_ptr(), offsets.data_ptr(), ids.data_ptr(), ids.numel(),
+                                                 C.byref(total), stream))
+        return JoinResult(counts=counts, offsets=offsets, ids=ids[:tot], total=tot, stats={})
+
 
 @dataclass
 class BatchResult:

@@ -147,6 +147,26 @@ int gdsj_result_copy(gdsj_handle *h, uint32_t *ids, uint64_t cap);
 
 int gdsj_get_stats(gdsj_handle *h, gdsj_stats *out);
 
+/* All-pairs join: replaces the reference's search mode 1
+ * (makeDistanceTableGPUBruteForce GPU.cu:1285-1480, kernelBruteForce
+ * kernel.cu:486-515), which tests every ordered pair (i, j), i included,
+ * and only counts them.  Every pair is tested with the join's distance
+ * arithmetic for p->ilp / p->shortcircuit (ILP = 0 is the reference
+ * kernel's sequential sum) and the predicate sqrt(acc) <= epsilon, in the
+ * input's dimension order (REORDER does not apply, as in mode 1).  Writes
+ * counts[n], offsets[n+1] (NULL = internal) and *total; with d_ids, the
+ * rows, each ascending (GDSJ_EOVERFLOW when total > ids_cap).  The
+ * all-pairs result differs from the grid join's only by pairs within
+ * epsilon whose cells are not adjacent (rounding of the cell coordinates). */
+int gdsj_bruteforce_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t dims,
+                           int dtype, double epsilon, const gdsj_params *p,
+                           uint32_t *d_counts, uint64_t *d_offsets, uint32_t *d_ids,
+                           uint64_t ids_cap, uint64_t *total, void *stream);
+/* Host-buffer form: counts[n] on the host and the total (the CLI's mode 1). */
+int gdsj_bruteforce_host(gdsj_handle *h, const void *data, uint32_t n, uint32_t dims,
+                         int dtype, double epsilon, const gdsj_params *p,
+                         uint32_t *counts, uint64_t *total);
+
 /* Largest x with sqrt_rn(x) <= eps, the threshold that replaces the
  * reference's per-pair square root (kernel.cu:361,364). */
 float gdsj_eps_threshold_f32(float eps);

@@ -1,5 +1,6 @@
 """Dataset import (gdsj_csv_load, import_dataset.cpp:49-95) and the
-command-line join (gdsjoin_main <file> <eps> <dim> 3, main.cu:66-331)."""
+command-line join (gdsjoin_main <file> <eps> <dim> 3|1, main.cu:66-331),
+with the table and outlier-score files (main.cu:544-626)."""
 import os
 import re
 import subprocess
@@ -110,3 +111,53 @@ def test_cli_print_matches_oracle(gpu, tmp_path):
         c = int(ref["counts"][i])
         assert got == sorted(ref["ids"][off:off + c].tolist())
         off += c
+
+
+def _write_points(path, x):
+    path.write_text("\n".join(",".join(repr(float(v)) for v in row) for row in x) + "\n")
+
+
+@pytest.mark.gpu
+def test_cli_table_and_outlier_files(gpu, tmp_path):
+    rng = np.random.default_rng(12)
+    x = rng.uniform(0, 10, (400, 3)).astype(np.float32)
+    p = tmp_path / "pts.txt"
+    _write_points(p, x)
+    r = subprocess.run([CLI, str(p), "1.5", "3", "3", "--table-file", "--outlier-scores", "--reorder", "0"],
+                       capture_output=True, text=True, cwd=str(tmp_path), timeout=120)
+    assert r.returncode == 0, r.stderr
+    xs = _lib.csv_load(str(p)).reshape(-1, 3)
+    ref = O.join(xs, float(np.float32(1.5)), k=3, ilp=8, shortcircuit=1, reorder=0)
+    counts = ref["counts"].astype(np.int64)
+    # DSSJ_out.txt: header, then per point id its ascending ids, each followed by ", "
+    lines = (tmp_path / "DSSJ_out.txt").read_text().split("\n")
+    assert lines[0] == "#data point (line is the point id), neighbor point ids"
+    off = 0
+    for i in range(400):
+        c = int(counts[i])
+        want = "".join("%d, " % v for v in sorted(ref["ids"][off:off + c].tolist()))
+        assert lines[1 + i] == want
+        off += c
+    # DSSJ_outlier_scores.txt: count, rank by ascending count (ties by id)
+    sl = (tmp_path / "DSSJ_outlier_scores.txt").read_text().strip().split("\n")
+    assert sl[0].startswith("#data point (line is the point id), col0: Number of neighbors")
+    order = np.argsort(counts, kind="stable")
+    rank = np.empty(400, np.int64)
+    rank[order] = np.arange(400)
+    got = np.array([[int(t) for t in ln.split(",")] for ln in sl[1:]])
+    assert np.array_equal(got[:, 0], counts) and np.array_equal(got[:, 1], rank)
+
+
+@pytest.mark.gpu
+def test_cli_mode1_bruteforce_total(gpu, tmp_path):
+    rng = np.random.default_rng(13)
+    x = rng.uniform(0, 5, (500, 2)).astype(np.float32)
+    p = tmp_path / "pts.txt"
+    _write_points(p, x)
+    r = subprocess.run([CLI, str(p), "0.75", "2", "1"], capture_output=True, text=True, cwd=str(tmp_path),
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"Total neighbours in table: (\d+)", r.stdout)
+    xs = _lib.csv_load(str(p)).reshape(-1, 2)
+    ref = O.bruteforce(xs, float(np.float32(0.75)), ilp=0, shortcircuit=0, reorder=0)
+    assert m and int(m.group(1)) == int(ref["counts"].sum())
