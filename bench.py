@@ -198,6 +198,37 @@ def run_variant(sj, pts, eps, kw, steps, warmup, total=None):
     return total, 1e3 * wall / steps, per
 
 
+def rejoin_rate(pts, eps, kw, total, steps, warmup):
+    """Repeated joins of the same points over one grid (gdsj_rejoin_device):
+    the counting or slotted pass, scan and fill per step, the grid built
+    once.  Reported beside value (which rebuilds the grid every step)."""
+    import torch
+
+    from gpu_self_join_amd import SelfJoin
+    n = pts.shape[0]
+    dev = pts.device
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
+    ids = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    sj = SelfJoin(0)
+    try:
+        sj.join(pts, eps, out_ids=ids, counts=counts, offsets=offsets, stream=stream, **kw)
+        for _ in range(warmup):
+            sj.rejoin(out_ids=ids, counts=counts, offsets=offsets, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = sj.rejoin(out_ids=ids, counts=counts, offsets=offsets, stream=stream)
+            assert r.total == total
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+    finally:
+        sj.close()
+    return {"ms_per_join": round(ms, 4), "pairs_per_s": total / (ms * 1e-3), "steps": steps,
+            "path": "gdsj_rejoin_device: grid, runs and tiles built once, passes repeated"}
+
+
 def host_path(x, eps, kw, steps=2, warmup=1):
     """The drop-in's host-buffer path (gdsjoin): host points in, the counts
     and the table copied back over PCIe, the library's per-device handle
@@ -453,6 +484,7 @@ def main():
         "filter_error": filter_error(pts, eps, base) if d >= 16 else None,
         "roofline": roof,
         "host_path": None if args.no_host_path else host_path(x, eps, kws[best]),
+        "rejoin": rejoin_rate(pts, eps, kws[best], total, args.steps, args.warmup),
     }
     if args.no_cpu_baseline:
         out["cpu_baseline"] = None

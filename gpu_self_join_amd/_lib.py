@@ -33,7 +33,7 @@ EXPORTS = [
     "gdsj_mlp_param_count", "gdsj_mlp_feature_count", "gdsj_mlp_forward", "gdsj_mlp_grad",
     "gdsj_model_get", "gdsj_model_set",
     "gdsj_batch_create", "gdsj_batch_destroy", "gdsj_batch_autotune", "gdsj_batch_run", "gdsj_batch_get_stats",
-    "gdsj_csv_load", "gdsj_bruteforce_device", "gdsj_bruteforce_host",
+    "gdsj_csv_load", "gdsj_bruteforce_device", "gdsj_bruteforce_host", "gdsj_rejoin_device",
 ]
 
 ILP_AUTO = 0xFFFFFFFF
@@ -151,6 +151,7 @@ def load():
     L.gdsj_csv_load.argtypes = [C.c_char_p, i32, P, u64, C.POINTER(u64)]
     L.gdsj_bruteforce_device.argtypes = [P, P, u32, u32, i32, C.c_double, C.POINTER(GdsjParams), P, P, P, u64,
                                          C.POINTER(u64), P]
+    L.gdsj_rejoin_device.argtypes = [P, P, P, P, u64, C.POINTER(u64), P]
     L.gdsj_bruteforce_host.argtypes = [P, P, u32, u32, i32, C.c_double, C.POINTER(GdsjParams), P,
                                        C.POINTER(u64)]
     _lib = L

@@ -58,6 +58,7 @@ struct gdsj_handle {
     DevBuf bfXs;                  // all-pairs join: SoA copy of the points
     bool use_hm = false;
     bool slotted = false;
+    bool have_grid = false;  // a grid, runs and tiles of the last join (gdsj_rejoin_device)
     uint32_t novf = 0;
     gdsj_params p{};
     int mode = 0;
@@ -353,6 +354,9 @@ static int slotted_pass(gdsj_handle *h, JoinArgs &a, bool *used) {
     return GDSJ_OK;
 }
 
+static int join_passes(gdsj_handle *h, uint32_t n, uint32_t dims, uint32_t *d_counts, uint64_t *d_offsets,
+                       uint32_t *d_ids, uint64_t ids_cap, uint64_t *total);
+
 int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t dims, int dtype, double epsilon,
                      const gdsj_params *pin, uint32_t *d_counts, uint64_t *d_offsets, uint32_t *d_ids,
                      uint64_t ids_cap, uint64_t *total, void *stream) {
@@ -363,6 +367,7 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     }
     GDSJ_HIP(hipSetDevice(h->device));
     h->have_counts = h->have_table = false;
+    h->have_grid = false;
     h->slotted = false;
     h->novf = 0;
     GDSJ_TRY(resolve_params(pin, dims, &h->p));
@@ -408,7 +413,6 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     if (h->use_gram) {
         GDSJ_TRY(gram_prepare(h->gram, h->G, dtype == GDSJ_DTYPE_F64 ? 1 : 0, s));
         GDSJ_TRY(h->gramStats.ensure(64));
-        GDSJ_HIP(hipMemsetAsync(h->gramStats.p, 0, 24, s));
     }
     // hit masks: exact passes over precomputed runs, plain kernels, no unicomp
     // (auto: from 4 dims, where a pair's distance costs more than reading
@@ -447,6 +451,16 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
             return GDSJ_EUNSUPPORTED;
         }
     }
+    h->have_grid = true;
+    return join_passes(h, n, dims, d_counts, d_offsets, d_ids, ids_cap, total);
+}
+
+// The passes of a join over the handle's grid (built by the last
+// gdsj_join_device): counting or slotted pass, scan, fill, training.
+static int join_passes(gdsj_handle *h, uint32_t n, uint32_t dims, uint32_t *d_counts, uint64_t *d_offsets,
+                       uint32_t *d_ids, uint64_t ids_cap, uint64_t *total) {
+    hipStream_t s = h->stream;
+    if (h->use_gram) GDSJ_HIP(hipMemsetAsync(h->gramStats.p, 0, 24, s));
     uint64_t *offs = d_offsets;
     if (!offs) {
         GDSJ_TRY(h->offsets.ensure(((size_t)n + 1) * 8));
@@ -527,6 +541,24 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     h->st.ms_total = h->st.ms_grid + h->st.ms_estimate + h->st.ms_count + h->st.ms_scan + h->st.ms_compact +
                      h->st.ms_fill + h->st.ms_sort;
     return rc;
+}
+
+int gdsj_rejoin_device(gdsj_handle *h, uint32_t *d_counts, uint64_t *d_offsets, uint32_t *d_ids, uint64_t ids_cap,
+                       uint64_t *total, void *stream) {
+    if (!h || !total) return GDSJ_EINVAL;
+    if (!h->have_grid) {
+        set_error("gdsj_rejoin_device needs a preceding gdsj_join_device on this handle");
+        return GDSJ_ESTATE;
+    }
+    if (!d_counts && h->G.n) return GDSJ_EINVAL;
+    GDSJ_HIP(hipSetDevice(h->device));
+    h->have_counts = h->have_table = false;
+    h->slotted = false;
+    h->novf = 0;
+    h->stream = stream ? (hipStream_t)stream : h->own;
+    memset(&h->st, 0, sizeof(h->st));
+    GDSJ_HIP(hipEventRecord(h->ev[0], h->stream));
+    return join_passes(h, h->G.n, h->G.d, d_counts, d_offsets, d_ids, ids_cap, total);
 }
 
 // All-pairs join (the reference's search mode 1, GPU.cu:1285-1480,

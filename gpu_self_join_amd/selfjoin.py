@@ -98,6 +98,32 @@ class SelfJoin:
             _lib.check(rc)
         return JoinResult(counts=counts, offsets=offsets, ids=out_ids[:tot], total=tot, stats=self.stats())
 
+    def rejoin(self, out_ids=None, counts=None, offsets=None, stream=None, n=None):
+        """Runs the last join again over the grid it built (the points must
+        be unchanged since): the counting or slotted pass, scan and fill.
+        n: the number of points of that join (for the output tensors)."""
+        torch = _torch()
+        dev = torch.device("cuda", self.device)
+        if counts is None or offsets is None:
+            if n is None:
+                raise ValueError("pass counts and offsets, or n")
+            counts = torch.empty(n, dtype=torch.int32, device=dev) if counts is None else counts
+            offsets = torch.empty(n + 1, dtype=torch.int64, device=dev) if offsets is None else offsets
+        cap = 0 if out_ids is None else out_ids.numel()
+        ids_ptr = None if out_ids is None else out_ids.data_ptr()
+        total = C.c_uint64(0)
+        if stream is None:
+            stream = torch.cuda.current_stream(dev).cuda_stream
+        rc = self.L.gdsj_rejoin_device(self.h, counts.data_ptr(), offsets.data_ptr(), ids_ptr, cap, C.byref(total),
+                                       stream)
+        tot = int(total.value)
+        if rc == _lib.GDSJ_EOVERFLOW or (out_ids is None and rc == _lib.GDSJ_OK):
+            out_ids = torch.empty(max(tot, 1), dtype=torch.int32, device=dev)
+            _lib.check(self.L.gdsj_fill_device(self.h, out_ids.data_ptr(), out_ids.numel()))
+        else:
+            _lib.check(rc)
+        return JoinResult(counts=counts, offsets=offsets, ids=out_ids[:tot], total=tot, stats=self.stats())
+
     def bruteforce(self, points, eps, params=None, stream=None, **kw):
         """All-pairs join of `points` (the reference's search mode 1,
         GPU.cu:1285-1480): every ordered pair tested with the join's

@@ -137,6 +137,16 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
 /* Completes the table of the last join on h into d_ids (cap >= total). */
 int gdsj_fill_device(gdsj_handle *h, uint32_t *d_ids, uint64_t ids_cap);
 
+/* Runs the last gdsj_join_device of h again over the grid, neighbor runs and
+ * tiles it built, on the same points (which must be unchanged since), with
+ * the same epsilon and parameters: the counting (or slotted) pass, the scan
+ * and the fill.  Replaces nothing in the reference, which rebuilds the index
+ * on every call (GDSJoinPy main.cu:428-450); for repeated joins of one
+ * dataset.  Outputs as gdsj_join_device; GDSJ_ESTATE without a preceding
+ * successful join. */
+int gdsj_rejoin_device(gdsj_handle *h, uint32_t *d_counts, uint64_t *d_offsets, uint32_t *d_ids,
+                       uint64_t ids_cap, uint64_t *total, void *stream);
+
 /* Host-buffer join (the path GDSJoinPy takes): copies data in, runs the
  * join, writes counts[n] to host, keeps the table on the handle for
  * gdsj_result_copy(). */

@@ -183,6 +183,45 @@ def test_hitmask_rejected_where_unsupported(sj):
         _gpu_join(sj, x, 1.0, k=2, estimator=1, hitmask=1)
 
 
+@pytest.mark.parametrize("d,k,est", [(2, 2, 0), (6, 6, 0), (6, 6, 1), (20, 6, 0)])
+def test_rejoin_reuses_the_grid(gpu, d, k, est):
+    """gdsj_rejoin_device reruns the passes over the last join's grid: the
+    same table as the join (byte for byte on the exact path), no grid build."""
+    import torch
+    from gpu_self_join_amd import SelfJoin
+    rng = np.random.default_rng(40 + d)
+    x = (rng.random((15000, d)) * (4.0 if d > 2 else 60.0)).astype(np.float32)
+    eps = 0.9 if d > 2 else 1.0
+    sj = SelfJoin(0)
+    try:
+        t = torch.from_numpy(x).cuda()
+        kw = dict(k=k, ilp=8, shortcircuit=1, reorder=1, estimator=est)
+        r1 = sj.join(t, eps, **kw)
+        c1, i1 = r1.counts.cpu().numpy(), r1.ids.cpu().numpy()
+        for _ in range(2):
+            r2 = sj.rejoin(n=len(x))
+            c2, i2 = r2.counts.cpu().numpy(), r2.ids.cpu().numpy()
+            assert np.array_equal(c1, c2)
+            if est == 0:
+                assert np.array_equal(i1, i2)
+            assert_csr_equal(c2.view(np.uint32), i2.view(np.uint32), c1.view(np.uint32), i1.view(np.uint32))
+            assert r2.stats["ms_grid"] < max(0.05, 0.5 * r1.stats["ms_grid"])
+        ref = O.join(x, eps, k=k, ilp=8, shortcircuit=1, reorder=1)
+        assert_csr_equal(c2.view(np.uint32), i2.view(np.uint32), ref["counts"], ref["ids"])
+    finally:
+        sj.close()
+
+
+def test_rejoin_needs_a_join(gpu):
+    from gpu_self_join_amd import SelfJoin
+    sj = SelfJoin(0)
+    try:
+        with pytest.raises(_lib.GdsjError):
+            sj.rejoin(n=10)
+    finally:
+        sj.close()
+
+
 def test_edge_single_point(sj):
     x = np.array([[1.0, 2.0, 3.0]], np.float32)
     c, ids, r = _gpu_join(sj, x, 0.5, k=3)
