@@ -256,7 +256,9 @@ static void set_slot_args(gdsj_handle *h, JoinArgs &a) {
     a.ovfCount = h->scalars.as<uint32_t>();
 }
 
-static int do_fill(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
+// Queues the table's passes (compaction and fills) on the handle's stream;
+// fill_finish waits for them.  Writes stop at cap in any case.
+static int fill_enqueue(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
     hipStream_t s = h->stream;
     JoinArgs a = make_args(h);
     a.offsets = h->cur_offsets;
@@ -289,6 +291,12 @@ static int do_fill(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
         GDSJ_HIP(hipEventRecord(h->ev[7], s));
     }
     GDSJ_HIP(hipEventRecord(h->ev[5], s));
+    return GDSJ_OK;
+}
+
+// Sorts the rows when asked, waits for the stream and checks the fill.
+static int fill_finish(gdsj_handle *h, uint32_t *d_ids) {
+    hipStream_t s = h->stream;
     if (h->p.sort_neighbors)
         GDSJ_TRY(sort_rows(h->cur_offsets, h->G.n, d_ids, s));
     GDSJ_HIP(hipEventRecord(h->ev[6], s));
@@ -308,6 +316,11 @@ static int do_fill(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
     }
     h->st.ms_sort = ms_between(h->ev[5], h->ev[6]);
     return GDSJ_OK;
+}
+
+static int do_fill(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
+    GDSJ_TRY(fill_enqueue(h, d_ids, cap));
+    return fill_finish(h, d_ids);
 }
 
 // Single pass with per-tile result slots sized by an estimator.  Returns
@@ -491,6 +504,11 @@ static int join_passes(gdsj_handle *h, uint32_t n, uint32_t dims, uint32_t *d_co
     GDSJ_HIP(hipEventRecord(h->ev[4], s));
     uint64_t *candSum = h->status.as<uint64_t>() + 4;
     GDSJ_TRY(sum_u64(a.tileWork, h->G.ntiles, candSum, s));
+    // exact passes into a caller's buffer: the fill is queued behind the
+    // scan before the total is known (its writes stop at ids_cap), so the
+    // join waits for the device once
+    const bool queued = d_ids != nullptr && !slotted;
+    if (queued) GDSJ_TRY(fill_enqueue(h, d_ids, ids_cap));
     uint64_t hres[3] = {0, 0, 0}, gst[3] = {0, 0, 0};
     if (h->use_gram) GDSJ_HIP(hipMemcpyAsync(gst, h->gramStats.p, 24, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipMemcpyAsync(&hres[0], offs + n, 8, hipMemcpyDeviceToHost, s));
@@ -527,6 +545,8 @@ static int join_passes(gdsj_handle *h, uint32_t n, uint32_t dims, uint32_t *d_co
         set_error("result has %llu pairs, output buffer holds %llu", (unsigned long long)tot,
                   (unsigned long long)ids_cap);
         rc = GDSJ_EOVERFLOW;
+    } else if (queued) {
+        GDSJ_TRY(fill_finish(h, d_ids));
     } else {
         GDSJ_TRY(do_fill(h, d_ids, ids_cap));
     }
