@@ -69,7 +69,7 @@ struct gdsj_handle {
     uint32_t *cur_counts = nullptr;
     uint64_t *cur_offsets = nullptr;
     gdsj_stats st{};
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[9] = {};  // 8: start of the row sort
 };
 
 extern "C" {
@@ -140,7 +140,7 @@ int gdsj_create(int device, gdsj_handle **out) {
         delete h;
         return GDSJ_EDEVICE;
     }
-    for (int i = 0; i < 8; i++) (void)hipEventCreate(&h->ev[i]);
+    for (int i = 0; i < 9; i++) (void)hipEventCreate(&h->ev[i]);
     h->stream = h->own;
     gdsj_params_default(&h->p);
     *out = h;
@@ -165,7 +165,7 @@ int gdsj_destroy(gdsj_handle *h) {
     h->gram.release();
     h->gramStats.release();
     for (DevBuf *x : b) x->release();
-    for (int i = 0; i < 8; i++)
+    for (int i = 0; i < 9; i++)
         if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
     if (h->own) (void)hipStreamDestroy(h->own);
     delete h;
@@ -297,9 +297,11 @@ static int fill_enqueue(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
 // Sorts the rows when asked, waits for the stream and checks the fill.
 static int fill_finish(gdsj_handle *h, uint32_t *d_ids) {
     hipStream_t s = h->stream;
-    if (h->p.sort_neighbors)
+    if (h->p.sort_neighbors) {
+        GDSJ_HIP(hipEventRecord(h->ev[8], s));
         GDSJ_TRY(sort_rows(h->cur_offsets, h->G.n, d_ids, s));
-    GDSJ_HIP(hipEventRecord(h->ev[6], s));
+        GDSJ_HIP(hipEventRecord(h->ev[6], s));
+    }
     uint32_t flag = 0;
     GDSJ_HIP(hipMemcpyAsync(&flag, h->status.p, 4, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipStreamSynchronize(s));
@@ -314,7 +316,7 @@ static int fill_finish(gdsj_handle *h, uint32_t *d_ids) {
     } else {
         h->st.ms_fill = ms_between(h->ev[4], h->ev[5]);
     }
-    h->st.ms_sort = ms_between(h->ev[5], h->ev[6]);
+    h->st.ms_sort = h->p.sort_neighbors ? ms_between(h->ev[8], h->ev[6]) : 0.0;
     return GDSJ_OK;
 }
 
