@@ -108,7 +108,7 @@ def cpu_reference_paths(x, eps, budget_s=8.0, seed=321):
     return out
 
 
-EST = {"exact": 0, "learned": 1, "sample": 2, "plain": 0, "nostamp": 0}
+EST = {"exact": 0, "learned": 1, "sample": 2, "plain": 0, "nostamp": 0, "linearstamp": 0}
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
@@ -373,7 +373,7 @@ def main():
         base[key] = int(val)
     esz = np.dtype(w["dtype"]).itemsize
     if args.variants is None:
-        args.variants = "exact,nostamp,learned" if base.get("stamp") else "exact,learned"
+        args.variants = "exact,nostamp,linearstamp,learned" if base.get("stamp") else "exact,learned"
     results = {}
     kws = {}
     total = None
@@ -384,6 +384,9 @@ def main():
         steps, warmup = args.steps, args.warmup
         if vname == "learned":
             kw["train"] = 0  # trains on the first join's counts, then sizes slots
+        if vname == "linearstamp":
+            # unicomp with the lower-linear-id ownership (LINEARSTAMP)
+            kw["stamp"] = 2
         if vname == "nostamp":
             # the exact passes over the full neighborhood (STAMP only changes
             # which side evaluates a pair, not the table)

@@ -183,6 +183,10 @@ static int resolve_params(const gdsj_params *in, uint32_t dims, gdsj_params *out
         set_error("ILP=%u exceeds 64", out->ilp);
         return GDSJ_EINVAL;
     }
+    if (out->stamp > 2) {
+        set_error("stamp=%u: 0 off, 1 STAMP, 2 LINEARSTAMP", out->stamp);
+        return GDSJ_EINVAL;
+    }
     return GDSJ_OK;
 }
 
@@ -217,7 +221,7 @@ static JoinArgs make_args(gdsj_handle *h) {
     a.xcd_map = h->p.schedule == 2 ? 1u : 0u;
     a.prune = h->p.prune == 1 || (h->p.prune == 0 && h->G.k >= 4) ? 1u : 0u;
     if (h->stamp) {
-        a.stamp = 1;
+        a.stamp = h->p.stamp;
         a.fcounts = h->fcnt.as<uint32_t>();
         a.rcur = h->rcur.as<uint32_t>();
     }
@@ -443,7 +447,8 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
         bool built = false;
         const bool prune = h->p.prune == 1 || (h->p.prune == 0 && h->G.k >= 4);
         if (want_hm) GDSJ_TRY(h->maskW.ensure((size_t)h->G.ncell * 4 + 256));
-        GDSJ_TRY(build_runs(h->G, s, prune, h->stamp, &built, want_hm ? h->maskW.as<uint32_t>() : nullptr));
+        GDSJ_TRY(build_runs(h->G, s, prune, h->stamp ? (int)h->p.stamp : 0, &built,
+                            want_hm ? h->maskW.as<uint32_t>() : nullptr));
         h->G.haveRuns = built;
     }
     if (want_hm && h->G.haveRuns) {

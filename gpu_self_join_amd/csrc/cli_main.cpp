@@ -10,7 +10,8 @@
 // reference fixes dims, NUMINDEXEDDIM, DTYPE and the optimisation switches
 // at compile time (params.h); here they are options:
 //   --dtype float|double (default float)  --k K (default min(dim, 6))
-//   --ilp I  --shortcircuit 0|1  --reorder 0|1  --stamp 0|1  --queryreorder 0|1
+//   --ilp I  --shortcircuit 0|1  --reorder 0|1  --stamp 0|1  --linearstamp 0|1
+//   --queryreorder 0|1
 //   --estimator exact|learned|sample  --filter plain|gram|auto  --device D
 //   --print   print every point's sorted neighbor list (PRINTNEIGHBORTABLE
 //             with STAMP, main.cu:299-322)
@@ -36,7 +37,7 @@ static int usage() {
             "usage: gdsjoin_main <dataset file> <epsilon> <dims> <mode> [options]\n"
             "  mode 3: GPU grid self-join; mode 1: all-pairs GPU join (total only)\n"
             "  options: --dtype float|double --k K --ilp I --shortcircuit 0|1 --reorder 0|1\n"
-            "           --stamp 0|1 --queryreorder 0|1 --estimator exact|learned|sample\n"
+            "           --stamp 0|1 --linearstamp 0|1 --queryreorder 0|1 --estimator exact|learned|sample\n"
             "           --filter plain|gram|auto --device D --print --table-file --outlier-scores\n");
     return 2;
 }
@@ -130,7 +131,8 @@ int main(int argc, char **argv) {
         else if (!strcmp(a, "--ilp")) p.ilp = (uint32_t)atoi(v);
         else if (!strcmp(a, "--shortcircuit")) p.shortcircuit = (uint32_t)atoi(v);
         else if (!strcmp(a, "--reorder")) p.reorder = (uint32_t)atoi(v);
-        else if (!strcmp(a, "--stamp")) p.stamp = (uint32_t)atoi(v);
+        else if (!strcmp(a, "--stamp")) p.stamp = atoi(v) ? 1u : (p.stamp == 2 ? 2u : 0u);
+        else if (!strcmp(a, "--linearstamp")) p.stamp = atoi(v) ? 2u : (p.stamp == 1 ? 1u : 0u);
         else if (!strcmp(a, "--queryreorder")) p.queryreorder = (uint32_t)atoi(v);
         else if (!strcmp(a, "--device")) device = atoi(v);
         else if (!strcmp(a, "--estimator")) {
@@ -233,7 +235,7 @@ int main(int argc, char **argv) {
     std::ofstream stats("gpu_stats.txt", std::ios::app);
     stats << secs << ", " << fname << ", " << eps << ", " << total
           << ", GPUNUMDIM/NUMINDEXEDDIM/ILP/STAMP/SORT/REORDER/SHORTCIRCUIT/QUERYREORDER/DTYPE(float/double): "
-          << ndim << ", " << k << ", " << p.ilp << ", " << p.stamp << ", " << 0 << ", " << p.reorder << ", "
+          << ndim << ", " << k << ", " << p.ilp << ", " << (p.stamp == 1 ? 1 : 0) << ", " << 0 << ", " << p.reorder << ", "
           << p.shortcircuit << ", " << p.queryreorder << ", " << (dtype == GDSJ_DTYPE_F64 ? "double" : "float")
           << std::endl;
     stats.close();

@@ -59,7 +59,7 @@ struct JoinArgs {
     // tiles only and both directions are emitted; counts[] is accumulated
     // with atomics (zeroed before the counting pass), fcounts[q] = the
     // query's own-side count, rcur[j] = reverse entries placed so far
-    uint32_t stamp;
+    uint32_t stamp;  // 0 off, 1 STAMP, 2 LINEARSTAMP ownership
     uint32_t *fcounts;
     uint32_t *rcur;
     uint32_t *status;  // [0] set when a fill write would exceed cap
@@ -86,10 +86,11 @@ int join_pass(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t 
 // Builds the compacted neighbor-run list of every cell (runs, nruns,
 // cellCand = candidates per query of the cell) when it fits the memory cap;
 // returns false (and leaves the lists unset) otherwise.
-// stamp: only the neighbor cells each cell owns (unicomp).
+// stamp: only the neighbor cells each cell owns (unicomp: 1 STAMP, 2
+// LINEARSTAMP ownership, combo_decode in neighbors.h).
 // maskW (nullable): per cell, its queries' hit-mask words (saturated at
 // 2^32 - 1).
-int build_runs(Grid &G, hipStream_t s, bool prune, bool stamp, bool *built, uint32_t *maskW = nullptr);
+int build_runs(Grid &G, hipStream_t s, bool prune, int stamp, bool *built, uint32_t *maskW = nullptr);
 
 // out[0 .. *nout) = the tiles of list cut into sub-tiles of <= subq queries
 // (tileq: the tiles' maximum query count; out holds nlist * ceil(tileq/subq)).

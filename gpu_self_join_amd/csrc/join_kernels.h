@@ -289,7 +289,8 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
         uint2 run = make_uint2(0u, 0u);
         if (lane < nrw)
             run = pre ? a.runs[(uint64_t)cell * a.runStride + cw + lane]
-                      : combo_run(cw + lane, cc, glo, ghi, ga, a.cellStart, a.cellKey, a.hkeys, a.hvals, STAMP);
+                      : combo_run(cw + lane, cc, glo, ghi, ga, a.cellStart, a.cellKey, a.hkeys, a.hvals,
+                                  STAMP ? (int)a.stamp : 0);
         uint32_t len = run.y - run.x;
         uint32_t incl = len;
 #pragma unroll

@@ -31,12 +31,16 @@ __device__ __forceinline__ uint32_t cand_pos(uint32_t g, uint32_t nrw, uint32_t 
 // Cells whose squared gap to the tile's query box exceeds prune_lim are
 // skipped too: no pair with them can pass the predicate.
 //
-// stamp (unicomp, stamploops.h:2-201): keep only the cells this cell owns.
-// An adjacent pair of cells is owned by the one whose coordinate is odd in
-// the highest dimension where they differ, so each unordered pair is
-// evaluated from one side only: a row of dims 1..k-1 whose highest nonzero
-// offset is in dim j is kept iff c_j is odd; the cell's own row keeps the
-// dim-0 neighbors iff c_0 is odd (the own cell always).
+// stamp (unicomp): keep only the cells this cell owns, so each unordered
+// pair of adjacent cells is evaluated from one side only.
+//   1 (STAMP, stamploops.h:2-201): a pair is owned by the cell whose
+//     coordinate is odd in the highest dimension where they differ: a row of
+//     dims 1..k-1 whose highest nonzero offset is in dim j is kept iff c_j is
+//     odd; the cell's own row keeps the dim-0 neighbors iff c_0 is odd.
+//   2 (LINEARSTAMP, kernel.cu:272-289): a pair is owned by the cell with the
+//     smaller linear id: a row is kept iff its highest nonzero offset is +1
+//     (the row's linear ids exceed the cell's); the own row keeps x + 1.
+// The own cell is always kept.
 // The row of combination t: its linear id without dim 0 (base) and the
 // dim-0 range [x0, x1] left after the clamps and the pruning; live is false
 // when the whole row is skipped.
@@ -47,13 +51,14 @@ struct ComboRow {
 };
 
 __device__ __forceinline__ ComboRow combo_decode(uint32_t t, const uint32_t (&cc)[MAXK], const double *glo,
-                                                 const double *ghi, const GridArgs &ga, bool stamp) {
+                                                 const double *ghi, const GridArgs &ga, int stamp) {
     ComboRow w;
     w.base = 0;
     bool ok = true;
     double gap = 0.0;
     uint32_t tt = t;
-    int top = -1;  // highest dim (>= 1) with a nonzero offset
+    int top = -1;       // highest dim (>= 1) with a nonzero offset
+    uint32_t topd = 1;  // its offset digit (0: -1, 2: +1)
     for (uint32_t j = 1; j < ga.k; j++) {
         const uint32_t dlt = tt % 3u;
         tt /= 3u;
@@ -61,14 +66,19 @@ __device__ __forceinline__ ComboRow combo_decode(uint32_t t, const uint32_t (&cc
         ok &= (cj >= 0) && (cj < (int64_t)ga.nCells[j]);
         gap += dlt == 0 ? glo[j] : (dlt == 2 ? ghi[j] : 0.0);
         w.base += (uint64_t)cj * ga.mult[j];
-        if (dlt != 1) top = (int)j;
+        if (dlt != 1) {
+            top = (int)j;
+            topd = dlt;
+        }
     }
-    w.live = ok && !(gap > ga.prune_lim) && !(stamp && top >= 0 && !(cc[top] & 1u));
+    const bool owned = stamp == 1 ? !(top >= 0 && !(cc[top] & 1u)) : (stamp == 2 ? !(top >= 0 && topd == 0) : true);
+    w.live = ok && !(gap > ga.prune_lim) && owned;
     w.x0 = cc[0] == 0 ? 0u : cc[0] - 1u;
     w.x1 = cc[0] + 1u < ga.nCells[0] ? cc[0] + 1u : ga.nCells[0] - 1u;
     if (w.x0 < cc[0] && gap + glo[0] > ga.prune_lim) w.x0 = cc[0];
     if (w.x1 > cc[0] && gap + ghi[0] > ga.prune_lim) w.x1 = cc[0];
-    if (stamp && top < 0 && !(cc[0] & 1u)) w.x0 = w.x1 = cc[0];
+    if (stamp == 1 && top < 0 && !(cc[0] & 1u)) w.x0 = w.x1 = cc[0];
+    if (stamp == 2 && top < 0) w.x0 = cc[0];
     return w;
 }
 
@@ -77,7 +87,7 @@ __device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK
                                            const uint32_t *__restrict__ cellStart,
                                            const uint64_t *__restrict__ cellKey,
                                            const uint64_t *__restrict__ hkeys,
-                                           const uint32_t *__restrict__ hvals, bool stamp = false) {
+                                           const uint32_t *__restrict__ hvals, int stamp = 0) {
     const ComboRow w = combo_decode(t, cc, glo, ghi, ga, stamp);
     uint2 run = make_uint2(0u, 0u);
     if (!w.live) return run;

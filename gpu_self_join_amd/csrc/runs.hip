@@ -20,9 +20,9 @@ namespace gdsj {
 // join, independently of the cell, by a table entry {delta lo, delta hi,
 // info, 0}: delta = sum_j (d_j - 1) * mult[j] (two's complement), the
 // offset of the neighbor row's linear id from the cell's; info = ia |
-// ib << 12 | top << 24 with ia / ib the base-3 index of the digits of dims
-// 1..a / a+1..k-1 (a = (k-1)/2) and top the highest dim with a nonzero
-// offset (0: none).  Per cell, the squared gaps of the 3^a and 3^(k-1-a)
+// ib << 12 | top << 24 | d_top << 28 with ia / ib the base-3 index of the
+// digits of dims 1..a / a+1..k-1 (a = (k-1)/2), top the highest dim with a
+// nonzero offset (0: none) and d_top its digit.  Per cell, the squared gaps of the 3^a and 3^(k-1-a)
 // digit groups (+inf where a coordinate leaves the grid) sit in LDS, so a
 // combination's gap is two LDS reads and an add instead of a loop over the
 // dims.
@@ -34,7 +34,7 @@ static void combo_table(Grid &G, uint32_t *na, uint32_t *nb, uint32_t *split) {
     const uint32_t ncombo = pa * pb;
     G.hComboTab.resize(ncombo);
     for (uint32_t t = 0; t < ncombo; t++) {
-        uint32_t tt = t, ia = 0, ib = 0, top = 0, wa = 1, wb = 1;
+        uint32_t tt = t, ia = 0, ib = 0, top = 0, topd = 1, wa = 1, wb = 1;
         uint64_t delta = 0;
         for (uint32_t j = 1; j <= m; j++) {
             const uint32_t d = tt % 3u;
@@ -47,9 +47,13 @@ static void combo_table(Grid &G, uint32_t *na, uint32_t *nb, uint32_t *split) {
                 ib += d * wb;
                 wb *= 3u;
             }
-            if (d != 1) top = j;
+            if (d != 1) {
+                top = j;
+                topd = d;
+            }
         }
-        G.hComboTab[t] = make_uint4((uint32_t)delta, (uint32_t)(delta >> 32), ia | (ib << 12) | (top << 24), 0u);
+        G.hComboTab[t] = make_uint4((uint32_t)delta, (uint32_t)(delta >> 32),
+                                    ia | (ib << 12) | (top << 24) | (topd << 28), 0u);
     }
     *na = pa;
     *nb = pb;
@@ -128,14 +132,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R > 1 ? 4 :
             const uint32_t t = cw + (uint32_t)r * GDSJ_WAVE + lane;
             const uint4 e = tab[t < ncombo ? t : 0u];
             const double g = gA[e.z & 0xFFFu] + gB[(e.z >> 12) & 0xFFFu];
-            const uint32_t top = e.z >> 24;
+            const uint32_t top = (e.z >> 24) & 0xFu, topd = e.z >> 28;
+            // unicomp ownership (combo_decode's rules)
+            const bool owned = stamp == 1 ? !(top > 0 && !((odd >> top) & 1u))
+                                          : (stamp == 2 ? !(top > 0 && topd == 0) : true);
             w[r].base = rowbase + (((uint64_t)e.y << 32) | e.x);
-            w[r].live = t < ncombo && !(g > ga.prune_lim) && !(stamp && top > 0 && !((odd >> top) & 1u));
+            w[r].live = t < ncombo && !(g > ga.prune_lim) && owned;
             w[r].x0 = x0d;
             w[r].x1 = x1d;
             if (x0d < c0 && g + g0lo > ga.prune_lim) w[r].x0 = c0;
             if (x1d > c0 && g + g0hi > ga.prune_lim) w[r].x1 = c0;
-            if (stamp && top == 0 && !(c0 & 1u)) w[r].x0 = w[r].x1 = c0;
+            if (stamp == 1 && top == 0 && !(c0 & 1u)) w[r].x0 = w[r].x1 = c0;
+            if (stamp == 2 && top == 0) w[r].x0 = c0;
         }
         uint2 rr[R];
         combo_lookup<R>(w, ga, cellStart, cellKey, hkeys, hvals, rr);
@@ -172,17 +180,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R > 1 ? 4 :
 
 template <typename T, int R>
 static void launch_cell_runs(Grid &G, uint32_t blocks, uint32_t ncombo, uint32_t na, uint32_t nb, uint32_t split,
-                             bool prune, bool stamp, uint32_t *maskW, hipStream_t s) {
+                             bool prune, int stamp, uint32_t *maskW, hipStream_t s) {
     const size_t lds = (size_t)(256 / GDSJ_WAVE) * (na + nb) * sizeof(double);
     k_cell_runs<T, R><<<blocks, 256, lds, s>>>(G.xs.as<T>(), G.ld, G.ga, G.cellStart.as<uint32_t>(),
                                                G.cellCoord.as<uint32_t>(), G.cellKey.as<uint64_t>(),
                                                G.hkeys.as<uint64_t>(), G.hvals.as<uint32_t>(), G.comboTab.as<uint4>(),
-                                               ncombo, na, nb, split, G.runStride, prune ? 1 : 0, stamp ? 1 : 0,
+                                               ncombo, na, nb, split, G.runStride, prune ? 1 : 0, stamp,
                                                G.runs.as<uint2>(), G.nruns.as<uint32_t>(), G.cellCand.as<uint32_t>(),
                                                G.cellHist.as<float>(), maskW);
 }
 
-int build_runs(Grid &G, hipStream_t s, bool prune, bool stamp, bool *built, uint32_t *maskW) {
+int build_runs(Grid &G, hipStream_t s, bool prune, int stamp, bool *built, uint32_t *maskW) {
     *built = false;
     uint64_t ncombo = 1;
     for (uint32_t j = 1; j < G.k; j++) ncombo *= 3;

@@ -59,7 +59,9 @@ typedef struct gdsj_params {
     uint32_t ilp;           /* ILP: 0 = sequential FMA accumulator; else partials */
     uint32_t shortcircuit;  /* SHORTCIRCUIT (speed only) */
     uint32_t reorder;       /* REORDER: permute dims by descending variance */
-    uint32_t stamp;         /* STAMP / unicomp (speed only; both directions emitted) */
+    uint32_t stamp;         /* unicomp, each adjacent cell pair evaluated once and both directions
+                               emitted (speed only): 0 off, 1 STAMP (stamploops.h ownership),
+                               2 LINEARSTAMP (kernel.cu:272-289, the lower linear id owns) */
     uint32_t queryreorder;  /* QUERYREORDER: 1 heaviest tiles first; 2 by occupancy class
                                (powers of two), cell order within a class (speed only) */
     uint32_t filter;        /* GDSJ_FILTER_* */

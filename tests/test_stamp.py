@@ -1,5 +1,6 @@
-"""Unicomp (STAMP, stamploops.h:2-201): each pair of adjacent cells is
-evaluated from one side and both directions are emitted.  The result set is
+"""Unicomp (STAMP, stamploops.h:2-201, and LINEARSTAMP, kernel.cu:272-289):
+each pair of adjacent cells is evaluated from one side (the odd-coordinate
+owner, or the lower linear id) and both directions are emitted.  The result set is
 the one without unicomp (the pair's arithmetic is symmetric:
 fl(a - b) = -fl(b - a)); only the order inside a row may differ, so rows are
 compared sorted, and with sort_neighbors the tables are byte-identical."""
@@ -38,14 +39,15 @@ def _data(n, d, dt, kind, seed):
     return (c[rng.integers(0, 8, n)] + rng.standard_normal((n, d))).astype(dt)
 
 
+@pytest.mark.parametrize("mode", [1, 2], ids=["stamp", "linearstamp"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "n%d_d%d_k%d_%s_ilp%d_sc%d_r%d_%s" % (
     c[0], c[1], c[2], np.dtype(c[3]).name, c[4], c[5], c[6], c[7]))
-def test_stamp_equals_oracle_and_full_neighborhood(sj, case):
+def test_stamp_equals_oracle_and_full_neighborhood(sj, case, mode):
     n, d, k, dt, ilp, sc, reorder, kind = case
     x = _data(n, d, dt, kind, n + d)
     eps = float(np.asarray(0.35 if kind == "uniform" and d <= 3 else 0.9 + 0.15 * d, dtype=dt))
     kw = dict(k=k, ilp=ilp, shortcircuit=sc, reorder=reorder, filter=_lib.FILTER_PLAIN)
-    cs, is_, rs = _join(sj, x, eps, stamp=1, **kw)
+    cs, is_, rs = _join(sj, x, eps, stamp=mode, **kw)
     cf, if_, rf = _join(sj, x, eps, stamp=0, **kw)
     print(case, "pairs", rs.total, "candidates stamp %d full %d" % (rs.stats["candidates"], rf.stats["candidates"]),
           flush=True)
@@ -56,18 +58,19 @@ def test_stamp_equals_oracle_and_full_neighborhood(sj, case):
     ref = O.join(x, eps, k=k, ilp=ilp, shortcircuit=sc, reorder=reorder)
     assert_csr_equal(cs, is_, ref["counts"], ref["ids"])
     # sorted rows: byte-identical tables
-    c2, i2, _ = _join(sj, x, eps, stamp=1, sort_neighbors=1, **kw)
+    c2, i2, _ = _join(sj, x, eps, stamp=mode, sort_neighbors=1, **kw)
     c3, i3, _ = _join(sj, x, eps, stamp=0, sort_neighbors=1, **kw)
     assert np.array_equal(c2, c3) and np.array_equal(i2, i3)
 
 
-def test_stamp_dense_cells_span_several_tiles(sj):
+@pytest.mark.parametrize("mode", [1, 2], ids=["stamp", "linearstamp"])
+def test_stamp_dense_cells_span_several_tiles(sj, mode):
     # cells holding hundreds of points: a cell's queries span several tiles,
     # own-cell pairs are found from both of their tiles, cross-cell pairs once
     rng = np.random.default_rng(4)
     x = (rng.random((30000, 2)) * 4).astype(np.float32)
     eps = float(np.float32(0.5))
-    cs, is_, rs = _join(sj, x, eps, k=2, ilp=0, shortcircuit=0, reorder=0, stamp=1)
+    cs, is_, rs = _join(sj, x, eps, k=2, ilp=0, shortcircuit=0, reorder=0, stamp=mode)
     ref = O.join(x, eps, k=2)
     assert_csr_equal(cs, is_, ref["counts"], ref["ids"])
 
@@ -81,3 +84,16 @@ def test_stamp_ignored_with_estimator_and_gram(sj):
     cg, ig, rg = _join(sj, x, eps, k=4, stamp=1, filter=_lib.FILTER_GRAM)
     cp, ip, rp = _join(sj, x, eps, k=4, stamp=0, filter=_lib.FILTER_PLAIN)
     assert np.array_equal(cg, cp) and np.array_equal(ig, ip)
+
+
+def test_both_ownership_rules_evaluate_each_cell_pair_once(sj):
+    # without pruning (k < 4) both rules evaluate every own-cell pair and
+    # every adjacent cell pair exactly once: the same number of candidates
+    x = _data(20000, 3, np.float32, "uniform", 9)
+    eps = float(np.float32(0.6))
+    kw = dict(k=3, ilp=0, shortcircuit=0, reorder=0, filter=_lib.FILTER_PLAIN)
+    _, _, r1 = _join(sj, x, eps, stamp=1, **kw)
+    _, _, r2 = _join(sj, x, eps, stamp=2, **kw)
+    assert r1.stats["candidates"] == r2.stats["candidates"]
+    with pytest.raises(_lib.GdsjError):
+        _join(sj, x, eps, stamp=3, **kw)
