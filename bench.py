@@ -288,7 +288,8 @@ def bench_batched(args):
     kb = coord_bytes + 4 * npts
     achieved = kb / (t["ms"] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": "k_batch_join<count pass>", "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic("batched", "k_batch_join<count pass>")[0],
             "alg_bytes_per_launch": kb, "avg_launch_ms": round(t["ms"], 4)}
     out = {
         "metric": "exact result pairs/s (batched grid self-joins, %s)" % best,
