@@ -21,7 +21,7 @@ GDSJ_EUNSUPPORTED = -7
 DTYPE_F32 = 0
 DTYPE_F64 = 1
 
-FILTER_PLAIN, FILTER_GRAM, FILTER_AUTO = 0, 1, 2
+FILTER_PLAIN, FILTER_GRAM, FILTER_AUTO, FILTER_GRAM_BF16 = 0, 1, 2, 3
 EST_EXACT, EST_LEARNED, EST_SAMPLE = 0, 1, 2
 
 # every symbol include/gdsjoin.h declares

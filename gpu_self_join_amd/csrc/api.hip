@@ -408,8 +408,15 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     GDSJ_TRY(h->status.ensure(256));
     GDSJ_HIP(hipEventRecord(h->ev[0], s));
     // the MFMA Gram filter for dims >= 16 (AUTO) or on request
-    h->use_gram = (h->p.filter == GDSJ_FILTER_GRAM || h->p.filter == GDSJ_FILTER_AUTO) && gram_supported(dims);
-    if (h->p.filter == GDSJ_FILTER_GRAM && !h->use_gram) {
+    if (h->p.filter > GDSJ_FILTER_GRAM_BF16) {
+        set_error("filter=%u is not a GDSJ_FILTER_* mode", h->p.filter);
+        return GDSJ_EINVAL;
+    }
+    const bool gram_asked = h->p.filter == GDSJ_FILTER_GRAM || h->p.filter == GDSJ_FILTER_GRAM_BF16;
+    h->use_gram = (gram_asked || h->p.filter == GDSJ_FILTER_AUTO) && gram_supported(dims);
+    // operands: fp16 (one MFMA per k-step) unless bf16 hi + lo is asked for FP32 data
+    const int gram_lowp = (h->p.filter == GDSJ_FILTER_GRAM_BF16 && dtype == GDSJ_DTYPE_F32) ? 0 : 1;
+    if (gram_asked && !h->use_gram) {
         set_error("the Gram filter handles 16 to 128 dims (dims=%u)", dims);
         return GDSJ_EUNSUPPORTED;
     }
@@ -423,14 +430,14 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     }
     uint32_t tq = h->p.tile_queries ? h->p.tile_queries : 64;
     if (h->use_gram) {
-        const uint32_t gq = gram_tile_queries(dims, dtype == GDSJ_DTYPE_F64 ? 1 : 0);
+        const uint32_t gq = gram_tile_queries(dims, gram_lowp);
         if (!h->p.tile_queries || tq > gq) tq = gq;
     } else if (tq > 64) {
         tq = 64;  // the plain kernels keep one query per lane
     }
     GDSJ_TRY(grid_build(h->G, d_data, n, dims, dtype, epsilon, h->p.k, h->p.reorder, tq, h->p.queryreorder, s));
     if (h->use_gram) {
-        GDSJ_TRY(gram_prepare(h->gram, h->G, dtype == GDSJ_DTYPE_F64 ? 1 : 0, s));
+        GDSJ_TRY(gram_prepare(h->gram, h->G, gram_lowp, s));
         GDSJ_TRY(h->gramStats.ensure(64));
     }
     // hit masks: exact passes over precomputed runs, plain kernels, no unicomp

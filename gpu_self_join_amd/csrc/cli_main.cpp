@@ -12,7 +12,7 @@
 //   --dtype float|double (default float)  --k K (default min(dim, 6))
 //   --ilp I  --shortcircuit 0|1  --reorder 0|1  --stamp 0|1  --linearstamp 0|1
 //   --queryreorder 0|1
-//   --estimator exact|learned|sample  --filter plain|gram|auto  --device D
+//   --estimator exact|learned|sample  --filter plain|gram|gram-bf16|auto  --device D
 //   --print   print every point's sorted neighbor list (PRINTNEIGHBORTABLE
 //             with STAMP, main.cu:299-322)
 //   --table-file    write DSSJ_out.txt (printNeighborTable main.cu:544-565)
@@ -38,7 +38,8 @@ static int usage() {
             "  mode 3: GPU grid self-join; mode 1: all-pairs GPU join (total only)\n"
             "  options: --dtype float|double --k K --ilp I --shortcircuit 0|1 --reorder 0|1\n"
             "           --stamp 0|1 --linearstamp 0|1 --queryreorder 0|1 --estimator exact|learned|sample\n"
-            "           --filter plain|gram|auto --device D --print --table-file --outlier-scores\n");
+            "           --filter plain|gram|gram-bf16|auto --device D --print --table-file\n"
+            "           --outlier-scores\n");
     return 2;
 }
 
@@ -144,6 +145,7 @@ int main(int argc, char **argv) {
             if (!strcmp(v, "plain")) p.filter = GDSJ_FILTER_PLAIN;
             else if (!strcmp(v, "gram")) p.filter = GDSJ_FILTER_GRAM;
             else if (!strcmp(v, "auto")) p.filter = GDSJ_FILTER_AUTO;
+            else if (!strcmp(v, "gram-bf16")) p.filter = GDSJ_FILTER_GRAM_BF16;
             else return usage();
         } else {
             return usage();

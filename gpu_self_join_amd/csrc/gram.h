@@ -1,8 +1,9 @@
 // gram.h -- MFMA Gram-form distance filter for dims >= 16.
 //
 // Every point x (dims permuted, sorted order) is stored once per join as a
-// low-precision vector h = lp(sigma (x - o)) (bf16 for FP32 data, fp16 for
-// FP64 data; o = per-dim mean, sigma a power of two keeping fp16 in range)
+// low-precision vector h = lp(sigma (x - o)) (fp16, or bf16 in two terms for
+// FP32 data on request; o = per-dim mean, sigma a power of two keeping fp16
+// in range)
 // plus one float W = (1-c)/2 |h|^2 - (eps'/2 + e)^2, where e bounds |x' - h|
 // (measured exactly per point) and c bounds the FP32 accumulation error of
 // the MFMA dot products.  For a query q and candidate p the 32x32x16 MFMA

@@ -45,6 +45,8 @@ extern "C" {
 #define GDSJ_FILTER_PLAIN 0   /* direct differencing only */
 #define GDSJ_FILTER_GRAM 1    /* MFMA Gram-form filter + exact recheck */
 #define GDSJ_FILTER_AUTO 2    /* GRAM when dims >= 16, else PLAIN */
+#define GDSJ_FILTER_GRAM_BF16 3 /* GRAM with bf16 hi + lo operands (three MFMAs per k-step)
+                                   for FP32 data; GRAM, AUTO and FP64 data use fp16 */
 
 /* result-size estimators (sizing only; never change the result) */
 #define GDSJ_EST_EXACT 0      /* exact counting pass, then fill */

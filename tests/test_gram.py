@@ -1,5 +1,6 @@
 """The MFMA Gram-form filter (dims >= 16): identical tables to the plain join
-and to the oracle, including pairs at exactly eps, and filter statistics."""
+and to the oracle, including pairs at exactly eps, and filter statistics.
+Both operand formats: fp16 (the default) and, for FP32 data, bf16 hi + lo."""
 import numpy as np
 import pytest
 
@@ -34,14 +35,20 @@ CASES = [(3000, 16, 6, np.float32, 8, 1, 1), (3000, 32, 6, np.float32, 8, 1, 1),
          (2000, 16, 6, np.float64, 8, 1, 1), (2000, 64, 8, np.float64, 8, 1, 1), (1500, 40, 5, np.float64, 0, 0, 0)]
 
 
+MODES = [_lib.FILTER_GRAM, _lib.FILTER_GRAM_BF16]
+
+
+@pytest.mark.parametrize("mode", MODES, ids=["fp16", "bf16"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "n%d_d%d_k%d_%s_ilp%d_sc%d_r%d" % (
     c[0], c[1], c[2], np.dtype(c[3]).name, c[4], c[5], c[6]))
-def test_gram_equals_plain_and_oracle(sj, case):
+def test_gram_equals_plain_and_oracle(sj, case, mode):
     n, d, k, dt, ilp, sc, reorder = case
+    if mode == _lib.FILTER_GRAM_BF16 and dt == np.float64:
+        pytest.skip("bf16 operands apply to FP32 data only")
     x = _clustered(n, d, dt, n + d)
     eps = _eps(x, 25)
     kw = dict(k=k, ilp=ilp, shortcircuit=sc, reorder=reorder)
-    cg, ig, rg = _join(sj, x, eps, filter=_lib.FILTER_GRAM, **kw)
+    cg, ig, rg = _join(sj, x, eps, filter=mode, **kw)
     cp, ip, rp = _join(sj, x, eps, filter=_lib.FILTER_PLAIN, **kw)
     print("case", case, "total", rg.total, "rechecks", rg.stats["filter_rechecks"], "hits",
           rg.stats["filter_hits"], flush=True)
@@ -52,8 +59,9 @@ def test_gram_equals_plain_and_oracle(sj, case):
     assert_csr_equal(cg, ig, ref["counts"], ref["ids"])
 
 
+@pytest.mark.parametrize("mode", MODES, ids=["fp16", "bf16"])
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
-def test_gram_pairs_at_exactly_eps(sj, dt):
+def test_gram_pairs_at_exactly_eps(sj, dt, mode):
     # points on a 16-D lattice of spacing eps: neighbors at distance exactly
     # eps, sqrt(2) eps, ... -- the filter must send every boundary pair to the
     # exact recheck
@@ -67,17 +75,18 @@ def test_gram_pairs_at_exactly_eps(sj, dt):
     x = np.concatenate([x, x + dt(0.01 * eps)], axis=0)
     e = float(np.asarray(eps, dtype=dt))
     for ilp in (0, 8):
-        cg, ig, rg = _join(sj, x, e, k=4, ilp=ilp, shortcircuit=1, reorder=0, filter=_lib.FILTER_GRAM)
+        cg, ig, rg = _join(sj, x, e, k=4, ilp=ilp, shortcircuit=1, reorder=0, filter=mode)
         ref = O.join(x, e, k=4, ilp=ilp, shortcircuit=1, reorder=0)
         assert_csr_equal(cg, ig, ref["counts"], ref["ids"])
 
 
-def test_gram_large_coordinates(sj):
+@pytest.mark.parametrize("mode", MODES, ids=["fp16", "bf16"])
+def test_gram_large_coordinates(sj, mode):
     # far from the origin: the Gram form cancels catastrophically unless the
     # bound accounts for it; the result must still be exact
     x = _clustered(2000, 24, np.float32, 5) + np.float32(3.0e4)
     eps = _eps(x, 20)
-    cg, ig, rg = _join(sj, x, eps, k=6, ilp=8, shortcircuit=1, reorder=1, filter=_lib.FILTER_GRAM)
+    cg, ig, rg = _join(sj, x, eps, k=6, ilp=8, shortcircuit=1, reorder=1, filter=mode)
     ref = O.join(x, eps, k=6, ilp=8, shortcircuit=1, reorder=1)
     assert_csr_equal(cg, ig, ref["counts"], ref["ids"])
 
@@ -96,3 +105,14 @@ def test_gram_with_learned_slots(sj):
         h.close()
     ref = O.join(x, eps, k=6, ilp=8, shortcircuit=1, reorder=1)
     assert_csr_equal(c, ids, ref["counts"], ref["ids"])
+
+
+def test_gram_formats_agree_and_unknown_filter_rejected(sj):
+    x = _clustered(4000, 32, np.float32, 21, spread=1.5)
+    eps = _eps(x, 30)
+    kw = dict(k=6, ilp=8, shortcircuit=1, reorder=1)
+    c1, i1, r1 = _join(sj, x, eps, filter=_lib.FILTER_GRAM, **kw)
+    c2, i2, r2 = _join(sj, x, eps, filter=_lib.FILTER_GRAM_BF16, **kw)
+    assert np.array_equal(c1, c2) and np.array_equal(i1, i2)
+    with pytest.raises(_lib.GdsjError):
+        _join(sj, x, eps, filter=7, **kw)
