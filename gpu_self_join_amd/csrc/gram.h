@@ -25,6 +25,8 @@ struct GramData {
     DevBuf w;        // [n] float
     DevBuf n2;       // [n] double: N = |h + l|^2 (filter-error measurement)
     DevBuf scratch;  // per-dim means [d], max deviations [d], sigma
+    DevBuf xr;       // [n][xrs] DTYPE: the sorted points row-major (the exact rechecks' loads)
+    uint32_t xrs = 0;  // row stride: dims rounded up to a multiple of 16
     uint32_t dpad = 0;
     int lowp = 0;    // 0 bf16, 1 fp16
     int split = 0;   // 1: S = h.h' + h.l' + l.h' (three MFMAs per k-step)

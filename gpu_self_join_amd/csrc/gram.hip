@@ -20,6 +20,8 @@ namespace gdsj {
 void GramData::release() {
     h.release();
     l.release();
+    xr.release();
+    xrs = 0;
     w.release();
     n2.release();
     scratch.release();
@@ -141,6 +143,16 @@ __global__ void k_gram_convert(const T *__restrict__ xs, uint64_t ld, uint32_t n
     if (s == 0) sigOut[0] = sigma;
 }
 
+// row-major copy of the sorted points for the exact rechecks (padding zero)
+template <typename T>
+__global__ void k_rows(const T *__restrict__ xs, uint64_t ld, uint32_t n, uint32_t d, uint32_t xrs,
+                       T *__restrict__ xr) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)n * xrs) return;
+    const uint64_t p = i / xrs, l = i - p * xrs;
+    xr[i] = l < d ? xs[l * ld + p] : (T)0;
+}
+
 template <typename T, int LP>
 static int prepare_t(GramData &gd, const Grid &G, hipStream_t s) {
     const uint32_t n = G.n, d = G.d, dpad = gd.dpad;
@@ -149,6 +161,12 @@ static int prepare_t(GramData &gd, const Grid &G, hipStream_t s) {
     GDSJ_TRY(gd.w.ensure((size_t)n * 4 + 256));
     GDSJ_TRY(gd.n2.ensure((size_t)n * 8 + 256));
     GDSJ_TRY(gd.scratch.ensure((size_t)d * 16 + 256 + (size_t)d * GS * 8));
+    gd.xrs = (d + 15) / 16 * 16;
+    GDSJ_TRY(gd.xr.ensure((size_t)n * gd.xrs * sizeof(T) + 256));
+    {
+        const uint64_t tot = (uint64_t)n * gd.xrs;
+        k_rows<T><<<(uint32_t)((tot + 255) / 256), 256, 0, s>>>(G.xs.as<T>(), G.ld, n, d, gd.xrs, gd.xr.as<T>());
+    }
     double *mean = gd.scratch.as<double>();
     double *maxdev = mean + d;
     double *part = mean + 2 * (size_t)d + 8;  // after mean, maxdev and sigma

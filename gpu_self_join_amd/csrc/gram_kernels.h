@@ -61,7 +61,8 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                                               const typename Lp<LP>::scalar *__restrict__ H,
                                               const typename Lp<LP>::scalar *__restrict__ Lo,
                                               const float *__restrict__ W, const JoinArgs a, int mode,
-                                              unsigned long long *__restrict__ stats, const GramAux aux) {
+                                              unsigned long long *__restrict__ stats, const GramAux aux,
+                                              const T *__restrict__ XR, uint32_t xrs) {
     using vec = typename Lp<LP>::vec;
     constexpr uint32_t DP = KS * 16;
     constexpr int RB = rb_max(KS, SPLIT);
@@ -273,7 +274,8 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                 const uint32_t spc = shfl_u32(sp, col);  // lane col holds that candidate
                 if (e < stot) {
                     nrech++;
-                    const T dv = dist_mem_batch<T, 16>(xs, ld, q0 + r, spc, a.d, mode, a.ilp);
+                    const T dv = dist_row<T, 16>(XR + (uint64_t)(q0 + r) * xrs, XR + (uint64_t)spc * xrs, a.d,
+                                                 mode, a.ilp);
                     if (dv <= thr) {
                         nhit++;
                         anyhit = true;
@@ -377,14 +379,20 @@ inline int launch_gram(const GramData &gd, const JoinArgs &a, int mode, int pass
     const float *W = gd.w.as<float>();
     constexpr int SP = LP == 0 ? 1 : 0;
     const GramAux aux{gd.n2.as<double>(), gd.scratch.as<double>() + 2 * (size_t)a.d, err && stats ? 1u : 0u};
+    const T *XR = gd.xr.as<T>();
+    const uint32_t xrs = gd.xrs;
     if (pass == PASS_FILL)
-        k_gram<T, LP, KS, SP, PASS_FILL, false><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats, aux);
+        k_gram<T, LP, KS, SP, PASS_FILL, false><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats, aux, XR,
+                                                                      xrs);
     else if (pass == PASS_SLOT)
-        k_gram<T, LP, KS, SP, PASS_SLOT, false><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats, aux);
+        k_gram<T, LP, KS, SP, PASS_SLOT, false><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats, aux, XR,
+                                                                      xrs);
     else if (aux.err)  // measurement variant (counting pass only)
-        k_gram<T, LP, KS, SP, PASS_COUNT, true><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats, aux);
+        k_gram<T, LP, KS, SP, PASS_COUNT, true><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats, aux, XR,
+                                                                      xrs);
     else
-        k_gram<T, LP, KS, SP, PASS_COUNT, false><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats, aux);
+        k_gram<T, LP, KS, SP, PASS_COUNT, false><<<blocks, 256, 0, s>>>(xs, a.ids, H, Lo, W, a, mode, stats, aux, XR,
+                                                                       xrs);
     GDSJ_HIP(hipGetLastError());
     return GDSJ_OK;
 }

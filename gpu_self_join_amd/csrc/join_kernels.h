@@ -187,6 +187,49 @@ __device__ __forceinline__ T dist_mem_batch(const T *__restrict__ xs, uint64_t l
     return acc;
 }
 
+// dist_mem's arithmetic (in the same order) on two points stored as rows of
+// a row-major copy, padded to a multiple of C dims: a point's coordinates
+// share one or a few cache lines, where the SoA layout puts every dim in its
+// own line.  Used by the Gram filter's exact rechecks, whose candidates are
+// scattered.
+template <typename T, uint32_t C>
+__device__ __forceinline__ T dist_row(const T *__restrict__ qrow, const T *__restrict__ crow, uint32_t d, int mode,
+                                      uint32_t ilp) {
+    if (mode != ACC_ILP_NOSC) {
+        T acc = 0;
+        uint32_t g = 0;  // position of the dim within its group
+        for (uint32_t l0 = 0; l0 < d; l0 += C) {
+            T tv[C];
+#pragma unroll
+            for (uint32_t u = 0; u < C; u++) tv[u] = crow[l0 + u] - qrow[l0 + u];  // padded rows: in range
+#pragma unroll
+            for (uint32_t u = 0; u < C; u++) {
+                if (l0 + u < d) {
+                    if (mode == ACC_SEQ || g == 0) {
+                        acc = fma_rn(tv[u], tv[u], acc);
+                    } else {
+                        const T p = tv[u] * tv[u];
+                        acc = acc + p;
+                    }
+                    g = g + 1 == ilp ? 0u : g + 1;
+                }
+            }
+        }
+        return acc;
+    }
+    // ACC_ILP_NOSC: partial j accumulates dims j, j+ilp, ...; fold in j order
+    T acc = 0;
+    for (uint32_t j = 0; j < ilp; j++) {
+        T rj = 0;
+        for (uint32_t l = j; l < d; l += ilp) {
+            const T t = crow[l] - qrow[l];
+            rj = fma_rn(t, t, rj);
+        }
+        acc = (j == 0) ? rj : acc + rj;
+    }
+    return acc;
+}
+
 // One work tile of the join (one wave): the body shared by the per-join
 // kernel below and the persistent batched kernel (batch.hip).
 // CM: candidate chunks held in VGPRs (0: CMaxFor's default).
