@@ -87,11 +87,15 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     __shared__ double s_gap[256 / GDSJ_WAVE][2][MAXK];
     __shared__ uint32_t s_cnt[256 / GDSJ_WAVE][TQ];
     __shared__ unsigned long long s_cur[256 / GDSJ_WAVE][TQ];
-    __shared__ uint32_t s_hm[256 / GDSJ_WAVE][TQ];  // hit columns per row, current block
-    __shared__ uint32_t s_cid[256 / GDSJ_WAVE][32];  // ids of the current block's columns
     __shared__ __align__(16) float s_wq[256 / GDSJ_WAVE][TQ];  // query weights
-    constexpr uint32_t GQ = TQ * 32;                    // recheck queue: a block's every pair
-    __shared__ uint16_t s_q[256 / GDSJ_WAVE][GQ];       // survivors: row << 5 | column
+    // survivors waiting for their exact recheck, in block order (and so in
+    // candidate order per row): candidate position and row.  Rechecks run 64
+    // at a time, so that one memory round trip serves a wave of pairs rather
+    // than the few survivors of one block.  A row block adds at most 32 x 32
+    // entries; the queue holds them on top of a partial round.
+    constexpr uint32_t QCAP = 32 * 32 + GDSJ_WAVE;
+    __shared__ uint32_t s_qp[256 / GDSJ_WAVE][QCAP];
+    __shared__ uint8_t s_qr[256 / GDSJ_WAVE][QCAP];
     double *glo = s_gap[w][0];
     double *ghi = s_gap[w][1];
     uint32_t cc[MAXK];
@@ -113,7 +117,6 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     }
     for (uint32_t r = lane; r < TQ; r += GDSJ_WAVE) {
         s_cnt[w][r] = 0;
-        s_hm[w][r] = 0;
         s_cur[w][r] = (PASS == PASS_FILL && r < nq) ? a.offsets[ids[q0 + r]] : 0ull;
     }
     __builtin_amdgcn_wave_barrier();
@@ -148,7 +151,76 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     uint32_t nrech = 0, nhit = 0;
     double errmax = 0.0;
     bool overflow = false;
-    static_assert(TQ <= 128, "rows are owned by lanes l and l + 64");
+    static_assert(TQ <= 128, "rows fit the queue's 8-bit row field and the 7-bit peer match");
+    uint32_t qn = 0;  // queued survivors (uniform)
+    uint32_t *qp = s_qp[w];
+    uint8_t *qr = s_qr[w];
+    // rechecks the first cnt queued pairs (cnt a multiple of 64, or all of
+    // them at the end of the tile) and emits the hits in queue order; the
+    // rest move to the front
+    auto flush = [&](uint32_t cnt) {
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t e0 = 0; e0 < cnt; e0 += GDSJ_WAVE) {
+            const uint32_t e = e0 + lane;
+            const bool v = e < cnt;
+            const uint32_t pc = v ? qp[e] : 0u;
+            const uint32_t r = v ? (uint32_t)qr[e] : 0u;
+            bool hit = false;
+            if (v) {
+                nrech++;
+                const T dv = dist_row<T, 16>(XR + (uint64_t)(q0 + r) * xrs, XR + (uint64_t)pc * xrs, a.d, mode,
+                                             a.ilp);
+                hit = dv <= thr;
+            }
+            const uint64_t hm = ballot(hit);
+            if (hm == 0) continue;
+            nhit += hit ? 1u : 0u;
+            if (PASS == PASS_COUNT) {
+                if (hit) atomicAdd(&s_cnt[w][r], 1u);
+            } else if (PASS == PASS_SLOT) {
+                if (hit) {
+                    atomicAdd(&s_cnt[w][r], 1u);
+                    const uint32_t p = scur + mbcnt(hm);
+                    if (p < scap) {
+                        a.slotIds[sbase + p] = ids[pc];
+                        a.slotQ[sbase + p] = (uint8_t)r;
+                    }
+                }
+                scur += popc64(hm);
+            } else {
+                // lanes hitting the same row write in lane (= queue) order
+                uint64_t peers = hm;
+#pragma unroll
+                for (int b = 0; b < 7; b++) {
+                    const bool bit = (r >> b) & 1u;
+                    const uint64_t mb = ballot(bit);
+                    peers &= bit ? mb : ~mb;
+                }
+                const uint64_t below = peers & ltm;
+                uint64_t pos = 0;
+                if (hit) pos = s_cur[w][r] + popc64(below);
+                __builtin_amdgcn_wave_barrier();
+                if (hit) {
+                    if (pos < a.cap) a.out[pos] = ids[pc]; else overflow = true;
+                    if (below == 0) s_cur[w][r] += popc64(peers);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        const uint32_t rest = qn - cnt;  // < 64 by the callers
+        uint32_t tp = 0, tr = 0;
+        if (lane < rest) {
+            tp = qp[cnt + lane];
+            tr = qr[cnt + lane];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane < rest) {
+            qp[lane] = tp;
+            qr[lane] = (uint8_t)tr;
+        }
+        __builtin_amdgcn_wave_barrier();
+        qn = rest;
+    };
 
     for (uint32_t cw = 0; cw < nwin; cw += GDSJ_WAVE) {
         const uint32_t nrw = uniform(nwin - cw < (uint32_t)GDSJ_WAVE ? nwin - cw : (uint32_t)GDSJ_WAVE);
@@ -174,10 +246,23 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
         uint32_t sp_n;
         vec Bn[KS], Bln[KS];
         float wc_n;
+        // blocks advance through the candidate order, so the run of a
+        // block's first candidate is tracked by a scalar cursor (ri); runs
+        // average hundreds of candidates, and a block usually lies in one
+        // or two of them
+        uint32_t ri = 0;
         auto load_block = [&](uint32_t cbl) {
+            while (ri + 1 < nrw && (uint32_t)__builtin_amdgcn_readlane(rexcl, ri + 1) <= cbl) ri++;
             const uint32_t gl = cbl + row_l;
             const bool vl = gl < total;
-            uint32_t p = cand_pos(vl ? gl : 0u, nrw, rstart, rexcl);
+            uint32_t p = (uint32_t)__builtin_amdgcn_readlane(rstart, ri) +
+                         (gl - (uint32_t)__builtin_amdgcn_readlane(rexcl, ri));
+            // lanes past the end of run k take the next run's positions
+            for (uint32_t k = ri + 1; k < nrw; k++) {
+                const uint32_t nx = (uint32_t)__builtin_amdgcn_readlane(rexcl, k);
+                if (nx > cbl + 31u) break;
+                if (gl >= nx) p = (uint32_t)__builtin_amdgcn_readlane(rstart, k) + (gl - nx);
+            }
             p = vl ? p : fpos;
             sp_n = p;
 #pragma unroll
@@ -198,12 +283,11 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                 if (SPLIT) Bl[ks] = Bln[ks];
             }
             if (cb + 32 < total) load_block(cb + 32);
-            // per row block: MFMAs, then a 16-bit survivor mask per lane
-            uint32_t smv[RB];
-#pragma unroll
-            for (int rb = 0; rb < RB; rb++) {
-                smv[rb] = 0;
-                if (rb >= (int)nrb) continue;
+            // per row block: the MFMAs (those of the next row block are issued
+            // before this one's compares), then one wave mask per
+            // accumulator entry: bit l of the mask = the pair (row of the
+            // entry in lane l, column l & 31) survives the filter
+            auto gram_rb = [&](int rb) {
                 f32x16 acc = {};
 #pragma unroll
                 for (int ks = 0; ks < KS; ks++) {
@@ -213,132 +297,65 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                         acc = mfma32<LP>(Al[rb][ks], B[ks], acc);
                     }
                 }
-                uint32_t sm = 0;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const float4 wv = *reinterpret_cast<const float4 *>(&s_wq[w][rb * 32 + j * 8 + half * 4]);
-                    sm |= (acc[4 * j + 0] >= wv.x + wc) ? (1u << (4 * j + 0)) : 0u;
-                    sm |= (acc[4 * j + 1] >= wv.y + wc) ? (1u << (4 * j + 1)) : 0u;
-                    sm |= (acc[4 * j + 2] >= wv.z + wc) ? (1u << (4 * j + 2)) : 0u;
-                    sm |= (acc[4 * j + 3] >= wv.w + wc) ? (1u << (4 * j + 3)) : 0u;
-                }
-                smv[rb] = sm;
-                if (ERR && sm) {
-                    const double s2 = aux.sig[0] * aux.sig[0];
-#pragma unroll
-                    for (int i = 0; i < 16; i++) {
-                        if (!((sm >> i) & 1u)) continue;
-                        const uint32_t r = rb * 32 + (i / 4) * 8 + half * 4 + (i % 4);
-                        const double est = aux.nrm[q0 + r] + aux.nrm[sp] - 2.0 * (double)acc[i];
-                        double ex = 0.0;
-                        for (uint32_t l = 0; l < a.d; l++) {
-                            const double t = (double)xs[(uint64_t)l * ld + sp] - (double)xs[(uint64_t)l * ld + q0 + r];
-                            ex += t * t;
-                        }
-                        errmax = fmax(errmax, fabs(est - s2 * ex) / (s2 * a.ga.eps * a.ga.eps));
-                    }
-                }
-            }
-            // exact rechecks of the block's survivors; hits set column bits of
-            // their row.  The survivors are queued in LDS (row, column) and
-            // checked by all lanes at once, a pair per lane.
-            uint32_t ns = 0;
-#pragma unroll
-            for (int rb = 0; rb < RB; rb++) ns += __popc(smv[rb]);
-            uint32_t sincl = ns;
-#pragma unroll
-            for (int off = 1; off < GDSJ_WAVE; off <<= 1) {
-                const uint32_t o = __shfl_up(sincl, off, GDSJ_WAVE);
-                if (lane >= (uint32_t)off) sincl += o;
-            }
-            const uint32_t stot = uniform(shfl_u32(sincl, GDSJ_WAVE - 1));
-            if (stot == 0) continue;
-            bool anyhit = false;
-            uint16_t *qq = s_q[w];
-            uint32_t o = sincl - ns;
+                return acc;
+            };
+            f32x16 accn = gram_rb(0);
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
-                uint32_t sm = smv[rb];
-                while (sm) {
-                    const int i = __builtin_ctz(sm);
-                    sm &= sm - 1;
-                    const uint32_t r = rb * 32 + (i / 4) * 8 + half * 4 + (i % 4);
-                    qq[o++] = (uint16_t)((r << 5) | row_l);
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-            for (uint32_t e0 = 0; e0 < stot; e0 += GDSJ_WAVE) {
-                const uint32_t e = e0 + lane;
-                const uint32_t v = e < stot ? (uint32_t)qq[e] : 0u;
-                const uint32_t r = v >> 5, col = v & 31u;
-                const uint32_t spc = shfl_u32(sp, col);  // lane col holds that candidate
-                if (e < stot) {
-                    nrech++;
-                    const T dv = dist_row<T, 16>(XR + (uint64_t)(q0 + r) * xrs, XR + (uint64_t)spc * xrs, a.d,
-                                                 mode, a.ilp);
-                    if (dv <= thr) {
-                        nhit++;
-                        anyhit = true;
-                        atomicOr(&s_hm[w][r], 1u << col);
-                    }
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-            if (ballot(anyhit) == 0) continue;
-            if (half == 0) s_cid[w][row_l] = ids[sp];
-            __builtin_amdgcn_wave_barrier();
-            // emit: lane l owns rows l and l + 64; bits in column (= candidate) order
-            uint32_t hm0 = 0, hm1 = 0;
-            if (lane < TQ) { hm0 = s_hm[w][lane]; s_hm[w][lane] = 0; }
-            if (lane + 64 < TQ) { hm1 = s_hm[w][lane + 64]; s_hm[w][lane + 64] = 0; }
-            if (PASS == PASS_COUNT) {
-                if (hm0) s_cnt[w][lane] += __popc(hm0);
-                if (hm1) s_cnt[w][lane + 64] += __popc(hm1);
-            } else if (PASS == PASS_FILL) {
+                if (rb >= (int)nrb) break;
+                const f32x16 acc = accn;
+                if (rb + 1 < RB && rb + 1 < (int)nrb) accn = gram_rb(rb + 1);
+                // the common case, no survivor in the row block, costs a
+                // compare per entry and one branch
+                float4 wv[4];
+                uint64_t any = 0;
 #pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    uint32_t hm = t ? hm1 : hm0;
-                    if (!hm) continue;
-                    const uint32_t r = lane + 64 * t;
-                    uint64_t pos = s_cur[w][r];
-                    while (hm) {
-                        const int c = __builtin_ctz(hm);
-                        hm &= hm - 1;
-                        if (pos < a.cap) a.out[pos] = s_cid[w][c]; else overflow = true;
-                        pos++;
-                    }
-                    s_cur[w][r] = pos;
+                for (int j = 0; j < 4; j++) {
+                    wv[j] = *reinterpret_cast<const float4 *>(&s_wq[w][rb * 32 + j * 8 + half * 4]);
+                    any |= ballot(acc[4 * j + 0] >= wv[j].x + wc) | ballot(acc[4 * j + 1] >= wv[j].y + wc) |
+                           ballot(acc[4 * j + 2] >= wv[j].z + wc) | ballot(acc[4 * j + 3] >= wv[j].w + wc);
                 }
-            } else {
-                const uint32_t mine = __popc(hm0) + __popc(hm1);
-                if (hm0) s_cnt[w][lane] += __popc(hm0);
-                if (hm1) s_cnt[w][lane + 64] += __popc(hm1);
-                uint32_t incl = mine;
+                if (any == 0) continue;
+                if (qn + 32 * 32 > QCAP) flush(qn & ~(GDSJ_WAVE - 1u));
 #pragma unroll
-                for (int off = 1; off < GDSJ_WAVE; off <<= 1) {
-                    const uint32_t o = __shfl_up(incl, off, GDSJ_WAVE);
-                    if (lane >= (uint32_t)off) incl += o;
-                }
-                uint32_t p = scur + incl - mine;
+                for (int j = 0; j < 4; j++) {
 #pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    uint32_t hm = t ? hm1 : hm0;
-                    const uint32_t r = lane + 64 * t;
-                    while (hm) {
-                        const int c = __builtin_ctz(hm);
-                        hm &= hm - 1;
-                        if (p < scap) {
-                            a.slotIds[sbase + p] = s_cid[w][c];
-                            a.slotQ[sbase + p] = (uint8_t)r;
+                    for (int e = 0; e < 4; e++) {
+                        const int i = 4 * j + e;
+                        const float wq = e == 0 ? wv[j].x : (e == 1 ? wv[j].y : (e == 2 ? wv[j].z : wv[j].w));
+                        const bool keep = acc[i] >= wq + wc;
+                        const uint64_t m = ballot(keep);
+                        if (m == 0) continue;
+                        const uint32_t r = rb * 32 + j * 8 + half * 4 + e;
+                        if (keep) {
+                            // the row is formed here, on the rare survivor
+                            // path: hoisted, the 16 x RB rows would each hold
+                            // a register through the loop
+                            uint32_t hb = half * 4u;
+                            asm volatile("" : "+v"(hb));
+                            const uint32_t at = qn + mbcnt(m);
+                            qp[at] = sp;
+                            qr[at] = (uint8_t)(hb + (uint32_t)(rb * 32 + j * 8 + e));
+                            if (ERR) {
+                                const double s2 = aux.sig[0] * aux.sig[0];
+                                const double est = aux.nrm[q0 + r] + aux.nrm[sp] - 2.0 * (double)acc[i];
+                                double ex = 0.0;
+                                for (uint32_t l = 0; l < a.d; l++) {
+                                    const double t = (double)xs[(uint64_t)l * ld + sp] -
+                                                     (double)xs[(uint64_t)l * ld + q0 + r];
+                                    ex += t * t;
+                                }
+                                errmax = fmax(errmax, fabs(est - s2 * ex) / (s2 * a.ga.eps * a.ga.eps));
+                            }
                         }
-                        p++;
+                        qn += popc64(m);
                     }
                 }
-                scur += uniform(shfl_u32(incl, GDSJ_WAVE - 1));
             }
-            __builtin_amdgcn_wave_barrier();
+            if (qn >= GDSJ_WAVE) flush(qn & ~(GDSJ_WAVE - 1u));
         }
     }
+    if (qn) flush(qn);
     __builtin_amdgcn_wave_barrier();
     if (PASS != PASS_FILL) {
         for (uint32_t r = lane; r < nq; r += GDSJ_WAVE) a.counts[ids[q0 + r]] = s_cnt[w][r];
