@@ -299,12 +299,14 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                 }
                 return acc;
             };
-            f32x16 accn = gram_rb(0);
+            // two accumulators, alternating by row block
+            f32x16 acc2[2];
+            acc2[0] = gram_rb(0);
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
                 if (rb >= (int)nrb) break;
-                const f32x16 acc = accn;
-                if (rb + 1 < RB && rb + 1 < (int)nrb) accn = gram_rb(rb + 1);
+                if (rb + 1 < RB && rb + 1 < (int)nrb) acc2[(rb + 1) & 1] = gram_rb(rb + 1);
+                const f32x16 &acc = acc2[rb & 1];
                 // the common case, no survivor in the row block, costs a
                 // compare per entry and one branch
                 float4 wv[4];
