@@ -116,3 +116,36 @@ def test_gram_formats_agree_and_unknown_filter_rejected(sj):
     assert np.array_equal(c1, c2) and np.array_equal(i1, i2)
     with pytest.raises(_lib.GdsjError):
         _join(sj, x, eps, filter=7, **kw)
+
+
+@pytest.mark.parametrize("mode", MODES, ids=["fp16", "bf16"])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_gram_dense_cluster_every_pair_survives(sj, dt, mode):
+    # 700 points within about 0.2 of each other (eps = 1) among a sparse
+    # background: the cluster's tiles are 128 queries whose every candidate
+    # is a neighbor, so each 32 x 32 row block queues 1024 survivors (the
+    # recheck queue's largest load, flushed within the block) and the fill
+    # emits many hits of one row per recheck round
+    from gpu_self_join_amd import SelfJoin
+    if mode == _lib.FILTER_GRAM_BF16 and dt == np.float64:
+        pytest.skip("bf16 operands apply to FP32 data only")
+    rng = np.random.default_rng(11)
+    d = 32
+    core = 50.0 + 0.02 * rng.standard_normal((700, d))
+    bg = rng.random((1500, d)) * 100.0
+    x = np.concatenate([core, bg])[rng.permutation(2200)].astype(dt)
+    eps = 1.0
+    kw = dict(k=6, ilp=8, shortcircuit=1, reorder=1)
+    ref = O.join(x, eps, k=6, ilp=8, shortcircuit=1, reorder=1)
+    assert int(np.asarray(ref["counts"]).sum()) >= 700 * 700
+    h = SelfJoin(0)
+    try:
+        cp, ip, _ = _join(h, x, eps, filter=_lib.FILTER_PLAIN, **kw)
+        for est in (_lib.EST_EXACT, _lib.EST_LEARNED, _lib.EST_LEARNED):
+            cg, ig, rg = _join(h, x, eps, filter=mode, estimator=est, **kw)
+            print("dense cluster", np.dtype(dt).name, "estimator", est, "total", rg.total, "rechecks",
+                  rg.stats["filter_rechecks"], flush=True)
+            assert np.array_equal(cg, cp) and np.array_equal(ig, ip)  # same rows, same order
+            assert_csr_equal(cg, ig, ref["counts"], ref["ids"])
+    finally:
+        h.close()
