@@ -20,22 +20,35 @@ __device__ __forceinline__ void compact_tile(const JoinArgs &a, const uint32_t t
     __shared__ uint32_t scnt[256 / GDSJ_WAVE][COMPACT_TQ];
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x / GDSJ_WAVE;
+    // the tile's loads are issued together where they do not depend on each
+    // other: tiles average a few queries on some workloads, so their latency,
+    // not their bytes, sets the time
     const uint32_t used = uniform(a.slotUsed[tile]);
-    if (used > uniform(a.slotCap[tile])) return;  // overflowed: completed by the fill pass
+    const uint32_t cap = uniform(a.slotCap[tile]);
     const uint4 tl = a.tiles[tile];
-    const uint32_t q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
     const uint64_t sbase = uniform64(a.slotOff[tile]);
-    for (uint32_t q = lane; q < COMPACT_TQ; q += GDSJ_WAVE) {
-        scur[w][q] = q < nq ? a.offsets[a.ids[q0 + q]] : 0ull;
+    if (used > cap) return;  // overflowed: completed by the fill pass
+    const uint32_t q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    // first batch of entries, loaded beside the rows' offsets
+    bool vn = lane < used;
+    uint32_t qn = vn ? (uint32_t)a.slotQ[sbase + lane] : 0u;
+    uint32_t idn = vn ? a.slotIds[sbase + lane] : 0u;
+    for (uint32_t q = lane; q < nq; q += GDSJ_WAVE) {
+        scur[w][q] = a.offsets[a.ids[q0 + q]];
         scnt[w][q] = 0;
     }
     __builtin_amdgcn_wave_barrier();
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     bool overflow = false;
     for (uint32_t e = 0; e < used; e += GDSJ_WAVE) {
-        const bool valid = e + lane < used;
-        const uint32_t qi = valid ? (uint32_t)a.slotQ[sbase + e + lane] : 0u;
-        const uint32_t id = valid ? a.slotIds[sbase + e + lane] : 0u;
+        const bool valid = vn;
+        const uint32_t qi = qn, id = idn;
+        // the next batch is loaded while this one is placed
+        vn = e + GDSJ_WAVE + lane < used;
+        if (vn) {
+            qn = (uint32_t)a.slotQ[sbase + e + GDSJ_WAVE + lane];
+            idn = a.slotIds[sbase + e + GDSJ_WAVE + lane];
+        }
         uint64_t peers = ballot(valid);
 #pragma unroll
         for (int b = 0; b < 7; b++) {
@@ -50,7 +63,7 @@ __device__ __forceinline__ void compact_tile(const JoinArgs &a, const uint32_t t
             if (below == 0) scnt[w][qi] = popc64(peers);
         }
         __builtin_amdgcn_wave_barrier();
-        for (uint32_t q = lane; q < COMPACT_TQ; q += GDSJ_WAVE) {
+        for (uint32_t q = lane; q < nq; q += GDSJ_WAVE) {
             scur[w][q] += scnt[w][q];
             scnt[w][q] = 0;
         }
