@@ -48,6 +48,7 @@ struct gdsj_handle {
     DevBuf counts, offsets, ids, data, scanTmp, status, work;
     // slotted single pass (estimator-sized per-tile result slots)
     DevBuf slotCap, slotOff, slotUsed, slotOvf, slotIds, slotQ, scalars, estPred, estScratch, subTiles;
+    DevBuf winTiles, winCnt, winOff;  // window splits of the overflowed Gram tiles
     Estimator est;
     GramData gram;
     DevBuf gramStats;
@@ -160,7 +161,8 @@ int gdsj_destroy(gdsj_handle *h) {
     h->hmask.release();
     DevBuf *b[] = {&h->counts,  &h->offsets,  &h->ids,     &h->data,    &h->scanTmp, &h->status,
                    &h->work,    &h->slotCap,  &h->slotOff, &h->slotUsed, &h->slotOvf, &h->slotIds,
-                   &h->slotQ,   &h->scalars,  &h->estPred, &h->estScratch, &h->subTiles};
+                   &h->slotQ,   &h->scalars,  &h->estPred, &h->estScratch, &h->subTiles,
+                   &h->winTiles, &h->winCnt,  &h->winOff};
     h->est.release();
     h->gram.release();
     h->gramStats.release();
@@ -288,7 +290,35 @@ static int fill_enqueue(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
             a.tiles = h->subTiles.as<uint4>();
             a.tileList = nullptr;
             a.ntiles = nsub;
-            GDSJ_TRY(run_pass(h, a, PASS_FILL));
+            constexpr uint32_t WSPLIT = 32;
+            if (h->use_gram && !h->stamp && h->G.haveRuns && (uint64_t)nsub * WSPLIT <= 65536) {
+                // a Gram sub-tile still streams its cell's whole
+                // neighborhood, the largest of the join: its candidates are
+                // split WSPLIT ways as well (equal ranges of the cell's
+                // candidate order), counted, placed by a prefix over the
+                // splits, then filled (up to 65,536 split tiles, 100 MB of
+                // split counts and offsets)
+                const uint64_t nw = (uint64_t)nsub * WSPLIT;
+                GDSJ_TRY(h->winTiles.ensure(nw * 16 + 64));
+                GDSJ_TRY(h->winCnt.ensure(nw * SPLIT_ROWS * 4 + 64));
+                GDSJ_TRY(h->winOff.ensure(nw * SPLIT_ROWS * 8 + 64));
+                GDSJ_TRY(split_windows(h->subTiles.as<uint4>(), nsub, WSPLIT, h->winTiles.as<uint4>(), s));
+                JoinArgs b = a;
+                b.tiles = h->winTiles.as<uint4>();
+                b.ntiles = (uint32_t)nw;
+                b.winSplit = WSPLIT;
+                b.cellCand = h->G.cellCand.as<uint32_t>();
+                b.tileWork = nullptr;
+                b.splitCnt = h->winCnt.as<uint32_t>();
+                GDSJ_TRY(gram_join_pass(h->G, h->gram, b, h->mode, PASS_COUNT, nullptr, s, false));
+                GDSJ_TRY(split_offsets(b.tiles, nsub, WSPLIT, a.ids, a.offsets, b.splitCnt, h->winOff.as<uint64_t>(),
+                                       s));
+                b.splitCnt = nullptr;
+                b.splitOff = h->winOff.as<uint64_t>();
+                GDSJ_TRY(gram_join_pass(h->G, h->gram, b, h->mode, PASS_FILL, nullptr, s, false));
+            } else {
+                GDSJ_TRY(run_pass(h, a, PASS_FILL));
+            }
         }
     } else {
         GDSJ_TRY(run_pass(h, a, PASS_FILL));

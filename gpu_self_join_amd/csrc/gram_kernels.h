@@ -117,7 +117,9 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     }
     for (uint32_t r = lane; r < TQ; r += GDSJ_WAVE) {
         s_cnt[w][r] = 0;
-        s_cur[w][r] = (PASS == PASS_FILL && r < nq) ? a.offsets[ids[q0 + r]] : 0ull;
+        s_cur[w][r] = (PASS == PASS_FILL && r < nq)
+                          ? (a.splitOff ? a.splitOff[(uint64_t)tile * SPLIT_ROWS + r] : a.offsets[ids[q0 + r]])
+                          : 0ull;
     }
     __builtin_amdgcn_wave_barrier();
 
@@ -222,7 +224,16 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
         qn = rest;
     };
 
-    for (uint32_t cw = 0; cw < nwin; cw += GDSJ_WAVE) {
+    // candidates of this tile: all, or the (tl.w - 1)-th of a.winSplit
+    // equal ranges of the cell's candidate order (a.cellCand: its total)
+    uint32_t clo = 0, chi = 0xFFFFFFFFu;
+    if (a.winSplit && tl.w) {
+        const uint64_t tot = a.cellCand[cell], sidx = tl.w - 1;
+        clo = uniform((uint32_t)(sidx * tot / a.winSplit));
+        chi = uniform((uint32_t)((sidx + 1) * tot / a.winSplit));
+    }
+    uint32_t cbase = 0;  // candidate index of the window's first candidate
+    for (uint32_t cw = 0; cw < nwin && cbase < chi; cw += GDSJ_WAVE) {
         const uint32_t nrw = uniform(nwin - cw < (uint32_t)GDSJ_WAVE ? nwin - cw : (uint32_t)GDSJ_WAVE);
         uint2 run = make_uint2(0u, 0u);
         if (lane < nrw)
@@ -238,6 +249,11 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
         const uint32_t total = uniform(shfl_u32(incl, nrw - 1));
         if (total == 0) continue;
         work += total;
+        // this window's share of the tile's candidate range: [wlo, whi)
+        const uint32_t wlo = clo > cbase ? clo - cbase : 0u;
+        const uint32_t whi = chi - cbase < total ? chi - cbase : total;
+        cbase += total;
+        if (wlo >= whi) continue;
         const uint32_t rexcl = incl - len;
         const uint32_t rstart = run.x;
         const uint32_t fpos = uniform(cand_pos(0u, nrw, rstart, rexcl));
@@ -254,7 +270,7 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
         auto load_block = [&](uint32_t cbl) {
             while (ri + 1 < nrw && (uint32_t)__builtin_amdgcn_readlane(rexcl, ri + 1) <= cbl) ri++;
             const uint32_t gl = cbl + row_l;
-            const bool vl = gl < total;
+            const bool vl = gl >= wlo && gl < whi;
             uint32_t p = (uint32_t)__builtin_amdgcn_readlane(rstart, ri) +
                          (gl - (uint32_t)__builtin_amdgcn_readlane(rexcl, ri));
             // lanes past the end of run k take the next run's positions
@@ -272,8 +288,9 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
             }
             wc_n = vl ? W[p] : INFINITY;
         };
-        load_block(0u);
-        for (uint32_t cb = 0; cb < total; cb += 32) {
+        const uint32_t cb0 = wlo & ~31u;
+        load_block(cb0);
+        for (uint32_t cb = cb0; cb < whi; cb += 32) {
             const uint32_t sp = sp_n;
             const float wc = wc_n;
             vec B[KS], Bl[KS];
@@ -282,7 +299,7 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                 B[ks] = Bn[ks];
                 if (SPLIT) Bl[ks] = Bln[ks];
             }
-            if (cb + 32 < total) load_block(cb + 32);
+            if (cb + 32 < whi) load_block(cb + 32);
             // per row block: the MFMAs (those of the next row block are issued
             // before this one's compares), then one wave mask per
             // accumulator entry: bit l of the mask = the pair (row of the
@@ -360,7 +377,12 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     if (qn) flush(qn);
     __builtin_amdgcn_wave_barrier();
     if (PASS != PASS_FILL) {
-        for (uint32_t r = lane; r < nq; r += GDSJ_WAVE) a.counts[ids[q0 + r]] = s_cnt[w][r];
+        for (uint32_t r = lane; r < nq; r += GDSJ_WAVE) {
+            if (a.splitCnt)
+                a.splitCnt[(uint64_t)tile * SPLIT_ROWS + r] = s_cnt[w][r];
+            else
+                a.counts[ids[q0 + r]] = s_cnt[w][r];
+        }
         if (a.tileWork && lane == 0) a.tileWork[tile] = work * nq;
     }
     if (PASS == PASS_SLOT && lane == 0) {

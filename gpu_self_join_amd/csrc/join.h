@@ -73,7 +73,18 @@ struct JoinArgs {
     uint64_t *hmask;            // nullable: no masks
     const uint64_t *hmCell;
     const uint32_t *cellCand;   // candidates per query of each cell
+    // candidate splits (Gram kernel, over precomputed runs: the exact fill
+    // of overflowed tiles, whose neighborhoods are the largest):
+    // tiles[t].w = s + 1 restricts tile t to the s-th of winSplit equal
+    // ranges of its cell's candidate order (cellCand: the total).  The
+    // counting pass then writes row r's count to splitCnt[t * SPLIT_ROWS + r]
+    // and the fill pass starts row r at splitOff[t * SPLIT_ROWS + r].
+    uint32_t winSplit;          // 0: tiles are not split
+    uint32_t *splitCnt;
+    const uint64_t *splitOff;
 };
+
+constexpr uint32_t SPLIT_ROWS = 128;  // row stride of splitCnt / splitOff
 
 // mask words per query of a cell with cand candidates in nruns runs
 __host__ __device__ __forceinline__ uint32_t hmask_stride(uint32_t cand, uint32_t nruns) {
@@ -96,6 +107,16 @@ int build_runs(Grid &G, hipStream_t s, bool prune, int stamp, bool *built, uint3
 // (tileq: the tiles' maximum query count; out holds nlist * ceil(tileq/subq)).
 int split_tiles(const uint32_t *list, uint32_t nlist, const uint4 *tiles, uint32_t tileq, uint32_t subq,
                 uint4 *out, uint32_t *nout, hipStream_t s);
+
+// out[i * S + s] = sub-tile i restricted to the s-th of S candidate ranges
+// (w = s + 1; JoinArgs::winSplit).
+int split_windows(const uint4 *sub, uint32_t nsub, uint32_t S, uint4 *out, hipStream_t s);
+
+// off[(i * S + s) * SPLIT_ROWS + r] = where row r of window split s of
+// sub-tile i starts: the row's CSR offset plus the counts (cnt, same
+// layout) of the earlier splits.
+int split_offsets(const uint4 *tiles, uint32_t nsub, uint32_t S, const uint32_t *ids, const uint64_t *offsets,
+                  const uint32_t *cnt, uint64_t *off, hipStream_t s);
 
 // Moves every non-overflowed tile's slot entries into CSR rows.
 int compact_slots(const JoinArgs &a, hipStream_t s);

@@ -255,6 +255,44 @@ int split_tiles(const uint32_t *list, uint32_t nlist, const uint4 *tiles, uint32
     return GDSJ_OK;
 }
 
+__global__ void k_split_windows(const uint4 *__restrict__ sub, uint32_t nsub, uint32_t S, uint4 *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsub * S) return;
+    const uint4 t = sub[i / S];
+    out[i] = make_uint4(t.x, t.y, t.z, i % S + 1u);
+}
+
+int split_windows(const uint4 *sub, uint32_t nsub, uint32_t S, uint4 *out, hipStream_t s) {
+    if (nsub * S == 0) return GDSJ_OK;
+    k_split_windows<<<ceil_div(nsub * S, 256), 256, 0, s>>>(sub, nsub, S, out);
+    GDSJ_HIP(hipGetLastError());
+    return GDSJ_OK;
+}
+
+__global__ void k_split_offsets(const uint4 *__restrict__ tiles, uint32_t nsub, uint32_t S,
+                                const uint32_t *__restrict__ ids, const uint64_t *__restrict__ offsets,
+                                const uint32_t *__restrict__ cnt, uint64_t *__restrict__ off) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;  // (sub-tile, row)
+    if (i >= nsub * SPLIT_ROWS) return;
+    const uint32_t sub = i / SPLIT_ROWS, r = i % SPLIT_ROWS;
+    const uint4 t = tiles[(uint64_t)sub * S];
+    if (r >= t.z - t.y) return;
+    uint64_t base = offsets[ids[t.y + r]];
+    for (uint32_t k = 0; k < S; k++) {
+        const uint64_t at = ((uint64_t)sub * S + k) * SPLIT_ROWS + r;
+        off[at] = base;
+        base += cnt[at];
+    }
+}
+
+int split_offsets(const uint4 *tiles, uint32_t nsub, uint32_t S, const uint32_t *ids, const uint64_t *offsets,
+                  const uint32_t *cnt, uint64_t *off, hipStream_t s) {
+    if (nsub == 0) return GDSJ_OK;
+    k_split_offsets<<<ceil_div(nsub * SPLIT_ROWS, 256), 256, 0, s>>>(tiles, nsub, S, ids, offsets, cnt, off);
+    GDSJ_HIP(hipGetLastError());
+    return GDSJ_OK;
+}
+
 int compact_slots(const JoinArgs &a, hipStream_t s) {
     const uint32_t blocks = (a.ntiles + 3) / 4;
     if (blocks == 0) return GDSJ_OK;

@@ -149,3 +149,33 @@ def test_gram_dense_cluster_every_pair_survives(sj, dt, mode):
             assert_csr_equal(cg, ig, ref["counts"], ref["ids"])
     finally:
         h.close()
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_gram_forced_overflow_fill_split_by_candidates(sj, dt):
+    # a model that predicts no hits: nearly every tile overflows its slot and
+    # is filled exactly on sub-tiles whose candidates are split 32 ways
+    # (equal ranges of the cell's candidate order, cut inside windows and
+    # blocks), counted per split and placed by a prefix over the splits; the
+    # rows must come out as the plain join's, in the same order
+    import ctypes as C
+    from gpu_self_join_amd import SelfJoin
+    from oracle import mlp as M
+    L = _lib.load()
+    x = _clustered(6000, 32, dt, 77, spread=2.0)
+    eps = _eps(x, 40)
+    kw = dict(k=6, ilp=8, shortcircuit=1, reorder=1)
+    h = SelfJoin(0)
+    try:
+        cp, ip, _ = _join(h, x, eps, filter=_lib.FILTER_PLAIN, **kw)
+        p = np.zeros(M.P, np.float32)
+        assert L.gdsj_model_set(h.h, p.ctypes.data, M.P, C.c_double(0.0)) == 0
+        cg, ig, rg = _join(h, x, eps, filter=_lib.FILTER_GRAM, estimator=_lib.EST_LEARNED, train=2, **kw)
+        print("forced overflow", np.dtype(dt).name, "tiles", rg.stats["est_tiles"], "overflowed",
+              rg.stats["est_overflow_tiles"], "pairs", rg.total, flush=True)
+        assert rg.stats["est_overflow_tiles"] > 0
+        assert np.array_equal(cg, cp) and np.array_equal(ig, ip)  # same rows, same order
+    finally:
+        h.close()
+    ref = O.join(x, eps, k=6, ilp=8, shortcircuit=1, reorder=1)
+    assert_csr_equal(cg, ig, ref["counts"], ref["ids"])
