@@ -170,8 +170,9 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
             bool hit = false;
             if (v) {
                 nrech++;
-                const T dv = dist_row<T, 16>(XR + (uint64_t)(q0 + r) * xrs, XR + (uint64_t)pc * xrs, a.d, mode,
-                                             a.ilp);
+                // dims per batch of loads: fewer where the registers are short
+                constexpr uint32_t RC = (sizeof(T) == 8 || KS > 2) ? 4u : 16u;
+                const T dv = dist_row<T, RC>(XR + (uint64_t)(q0 + r) * xrs, XR + (uint64_t)pc * xrs, a.d, mode, a.ilp);
                 hit = dv <= thr;
             }
             const uint64_t hm = ballot(hit);
@@ -317,12 +318,19 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                 return acc;
             };
             // two accumulators, alternating by row block
+            // up to 32 dims: two accumulators alternating by row block, the
+            // next row block's MFMAs issued before this one's compares; more
+            // dims: one at a time, which keeps 64 dims at 3 waves per SIMD
+            constexpr bool PIPE = KS <= 2;
             f32x16 acc2[2];
-            acc2[0] = gram_rb(0);
+            if (PIPE) acc2[0] = gram_rb(0);
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
                 if (rb >= (int)nrb) break;
-                if (rb + 1 < RB && rb + 1 < (int)nrb) acc2[(rb + 1) & 1] = gram_rb(rb + 1);
+                if (!PIPE)
+                    acc2[rb & 1] = gram_rb(rb);
+                else if (rb + 1 < RB && rb + 1 < (int)nrb)
+                    acc2[(rb + 1) & 1] = gram_rb(rb + 1);
                 const f32x16 &acc = acc2[rb & 1];
                 // the common case, no survivor in the row block, costs a
                 // compare per entry and one branch
