@@ -170,8 +170,9 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
             bool hit = false;
             if (v) {
                 nrech++;
-                // dims per batch of loads: fewer where the registers are short
-                constexpr uint32_t RC = (sizeof(T) == 8 || KS > 2) ? 4u : 16u;
+                // dims per batch of loads (8 floats or 4 doubles: 8 VGPRs
+                // per row)
+                constexpr uint32_t RC = sizeof(T) == 8 ? 4u : 8u;
                 const T dv = dist_row<T, RC>(XR + (uint64_t)(q0 + r) * xrs, XR + (uint64_t)pc * xrs, a.d, mode, a.ilp);
                 hit = dv <= thr;
             }
@@ -318,20 +319,13 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                 return acc;
             };
             // two accumulators, alternating by row block
-            // up to 32 dims: two accumulators alternating by row block, the
-            // next row block's MFMAs issued before this one's compares; more
-            // dims: one at a time, which keeps 64 dims at 3 waves per SIMD
-            constexpr bool PIPE = KS <= 2;
-            f32x16 acc2[2];
-            if (PIPE) acc2[0] = gram_rb(0);
+            // one row block's accumulator at a time: issuing the next row
+            // block's MFMAs early costs 32 VGPRs, and without them the kernel
+            // runs 4 waves per SIMD at 32 dims and 3 at 64 (measured faster)
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
                 if (rb >= (int)nrb) break;
-                if (!PIPE)
-                    acc2[rb & 1] = gram_rb(rb);
-                else if (rb + 1 < RB && rb + 1 < (int)nrb)
-                    acc2[(rb + 1) & 1] = gram_rb(rb + 1);
-                const f32x16 &acc = acc2[rb & 1];
+                const f32x16 acc = gram_rb(rb);
                 // the common case, no survivor in the row block, costs a
                 // compare per entry and one branch
                 float4 wv[4];
