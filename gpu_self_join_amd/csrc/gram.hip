@@ -7,10 +7,11 @@
 // and used by all RB row blocks.  Output register i of lane l holds
 // S[row (i/4)*8 + (l/32)*4 + i%4][column l%32].  In the split form the
 // operands are h + l (two bf16 terms) and S = h.h' + h.l' + l.h'; the
-// neglected l.l' is inside the bound.  Survivors of S >= W_q + W_p are
-// rechecked exactly; hits are counted / written per row with the ballot +
-// mbcnt scheme of the plain join, in candidate order, so the rows come out
-// identical to the plain join's.
+// neglected l.l' is inside the bound.  Survivors of S >= W_q + W_p (one wave
+// mask per accumulator entry) queue in LDS across blocks and are rechecked
+// exactly, 64 at a time, from a row-major copy of the points; hits are
+// counted / written per row in queue order, which is candidate order, so the
+// rows come out identical to the plain join's.
 #include <math.h>
 
 #include "gram_kernels.h"
