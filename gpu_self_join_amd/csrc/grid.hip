@@ -234,6 +234,50 @@ __global__ void k_soa(const T *__restrict__ data, uint32_t n, uint32_t d, const 
     if (bad) atomicOr(nonfinite, 1u);
 }
 
+// The same as k_soa for a tile of TP sorted positions per block, transposed
+// through LDS: the rows are read with the threads along each row (coalesced;
+// one thread per point reads 64 rows per load instruction) and the SoA
+// columns are written as runs of TP.  tile = [d][TP + 1] (dynamic LDS).
+template <typename T, uint32_t TP>
+__global__ __launch_bounds__(256) void k_soa_tile(const T *__restrict__ data, uint32_t n, uint32_t d,
+                                                  const uint32_t *__restrict__ perm,
+                                                  const uint32_t *__restrict__ order, T *__restrict__ xs,
+                                                  uint64_t ld, uint32_t *__restrict__ ids,
+                                                  uint32_t *__restrict__ nonfinite) {
+    extern __shared__ __align__(16) unsigned char soa_lds[];
+    T *tile = reinterpret_cast<T *>(soa_lds);
+    __shared__ uint32_t s_id[TP];
+    __shared__ uint32_t s_perm[MAXD];
+    __shared__ uint32_t s_bad;
+    if (blockIdx.x == 0)  // NaN padding at positions n .. ld-1
+        for (uint64_t t = threadIdx.x; t < (ld - n) * d; t += blockDim.x)
+            xs[(t / (ld - n)) * ld + n + t % (ld - n)] = (T)__builtin_nan("");
+    const uint32_t s0 = blockIdx.x * TP;
+    const uint32_t np = n - s0 < TP ? n - s0 : TP;
+    if (threadIdx.x == 0) s_bad = 0;
+    for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
+        const uint32_t id = order[s0 + p];
+        s_id[p] = id;
+        ids[s0 + p] = id;
+    }
+    for (uint32_t l = threadIdx.x; l < d; l += blockDim.x) s_perm[l] = perm[l];
+    __syncthreads();
+    bool bad = false;
+    for (uint32_t t = threadIdx.x; t < np * d; t += blockDim.x) {
+        const uint32_t p = t / d, l = t - p * d;
+        const T v = data[(uint64_t)s_id[p] * d + s_perm[l]];
+        bad |= !isfinite(v);
+        tile[l * (TP + 1) + p] = v;
+    }
+    if (bad) atomicOr(&s_bad, 1u);
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < d * TP; t += blockDim.x) {
+        const uint32_t l = t / TP, p = t % TP;
+        if (p < np) xs[(uint64_t)l * ld + s0 + p] = tile[l * (TP + 1) + p];
+    }
+    if (threadIdx.x == 0 && s_bad) atomicOr(nonfinite, 1u);
+}
+
 // ------------------------------------------------- cell hash + coords --
 // Open-addressing table row -> first cell of the row, the replacement for
 // the per-candidate-cell binary search of the reference (kernel.cu:388-399).
@@ -456,7 +500,17 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     uint32_t *nonfinite = G.scalars.as<uint32_t>();
     uint32_t *idsOut = alt ? vA : vB;  // the value buffer the sort did not end in
     GDSJ_HIP(hipMemsetAsync(nonfinite, 0, 4, s));
-    k_soa<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, dperm, svals, G.xs.as<T>(), G.ld, idsOut, nonfinite);
+    if ((size_t)d * 65 * sizeof(T) <= 48 * 1024)
+        k_soa_tile<T, 64><<<ceil_div(n, 64), 256, (size_t)d * 65 * sizeof(T), s>>>(data, n, d, dperm, svals,
+                                                                                     G.xs.as<T>(), G.ld, idsOut,
+                                                                                     nonfinite);
+    else if ((size_t)d * 33 * sizeof(T) <= 48 * 1024)
+        k_soa_tile<T, 32><<<ceil_div(n, 32), 256, (size_t)d * 33 * sizeof(T), s>>>(data, n, d, dperm, svals,
+                                                                                     G.xs.as<T>(), G.ld, idsOut,
+                                                                                     nonfinite);
+    else
+        k_soa<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, dperm, svals, G.xs.as<T>(), G.ld, idsOut, nonfinite);
+    GDSJ_HIP(hipGetLastError());
     if (idsOut != G.ids.as<uint32_t>()) std::swap(G.ids, G.valB);
     uint32_t hdr[3] = {0, 0, 0};
     GDSJ_HIP(hipMemcpyAsync(&hdr[0], cidx + n, 4, hipMemcpyDeviceToHost, s));
