@@ -117,24 +117,36 @@ __global__ void k_gram_convert(const T *__restrict__ xs, uint64_t ld, uint32_t n
     double N = 0.0, N1 = 0.0, L = 0.0, E2 = 0.0, X2 = 0.0;
     typename Lp<LP>::scalar *hp = H + (uint64_t)s * dpad;
     typename Lp<LP>::scalar *lp = split ? Lo + (uint64_t)s * dpad : nullptr;
-    for (uint32_t l = 0; l < dpad; l++) {
-        double xp = 0.0;
-        if (l < d) xp = ((double)xs[(uint64_t)l * ld + s] - mean[l]) * sigma;
-        typename Lp<LP>::scalar hv, lv;
-        const double hd = lp_round<LP>(xp, hv);
-        double ld2 = 0.0;
-        hp[l] = hv;
-        if (split) {
-            ld2 = lp_round<LP>(xp - hd, lv);
-            lp[l] = lv;
+    // 8 dims per step: the operands leave as one 16-byte store per step
+    // (dpad is a multiple of 16)
+    for (uint32_t l0 = 0; l0 < dpad; l0 += 8) {
+        typename Lp<LP>::scalar hb[8], lb[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; u++) {
+            const uint32_t l = l0 + u;
+            double xp = 0.0;
+            if (l < d) xp = ((double)xs[(uint64_t)l * ld + s] - mean[l]) * sigma;
+            typename Lp<LP>::scalar hv, lv = {};
+            const double hd = lp_round<LP>(xp, hv);
+            double ld2 = 0.0;
+            hb[u] = hv;
+            if (split) ld2 = lp_round<LP>(xp - hd, lv);
+            lb[u] = lv;
+            const double a = hd + ld2;  // exact: ld2 is the leading bits of xp - hd
+            const double r = xp - a;
+            N += a * a;
+            N1 += (fabs(hd) + fabs(ld2)) * (fabs(hd) + fabs(ld2));
+            L += ld2 * ld2;
+            E2 += r * r;
+            X2 += xp * xp;
         }
-        const double a = hd + ld2;  // exact: ld2 is the leading bits of xp - hd
-        const double r = xp - a;
-        N += a * a;
-        N1 += (fabs(hd) + fabs(ld2)) * (fabs(hd) + fabs(ld2));
-        L += ld2 * ld2;
-        E2 += r * r;
-        X2 += xp * xp;
+        uint4 hv4, lv4;
+        __builtin_memcpy(&hv4, hb, 16);
+        *reinterpret_cast<uint4 *>(hp + l0) = hv4;
+        if (split) {
+            __builtin_memcpy(&lv4, lb, 16);
+            *reinterpret_cast<uint4 *>(lp + l0) = lv4;
+        }
     }
     const double e = sqrt(E2) * (1.0 + 0x1p-30) + sqrt(X2) * 0x1p-50;
     const double k = sigma * eps_p * 0.5 + e;
@@ -144,14 +156,25 @@ __global__ void k_gram_convert(const T *__restrict__ xs, uint64_t ld, uint32_t n
     if (s == 0) sigOut[0] = sigma;
 }
 
-// row-major copy of the sorted points for the exact rechecks (padding zero)
-template <typename T>
-__global__ void k_rows(const T *__restrict__ xs, uint64_t ld, uint32_t n, uint32_t d, uint32_t xrs,
-                       T *__restrict__ xr) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (uint64_t)n * xrs) return;
-    const uint64_t p = i / xrs, l = i - p * xrs;
-    xr[i] = l < d ? xs[l * ld + p] : (T)0;
+// row-major copy of the sorted points for the exact rechecks (padding zero),
+// TP points per block transposed through LDS (tile = [xrs][TP + 1]): the SoA
+// columns are read and the rows written as contiguous runs
+template <typename T, uint32_t TP>
+__global__ __launch_bounds__(256) void k_rows(const T *__restrict__ xs, uint64_t ld, uint32_t n, uint32_t d,
+                                              uint32_t xrs, T *__restrict__ xr) {
+    extern __shared__ __align__(16) unsigned char rows_lds[];
+    T *tile = reinterpret_cast<T *>(rows_lds);
+    const uint32_t s0 = blockIdx.x * TP;
+    const uint32_t np = n - s0 < TP ? n - s0 : TP;
+    for (uint32_t t = threadIdx.x; t < xrs * TP; t += blockDim.x) {
+        const uint32_t l = t / TP, p = t % TP;
+        tile[l * (TP + 1) + p] = (l < d && p < np) ? xs[(uint64_t)l * ld + s0 + p] : (T)0;
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < np * xrs; t += blockDim.x) {
+        const uint32_t p = t / xrs, l = t - p * xrs;
+        xr[(uint64_t)(s0 + p) * xrs + l] = tile[l * (TP + 1) + p];
+    }
 }
 
 template <typename T, int LP>
@@ -164,9 +187,14 @@ static int prepare_t(GramData &gd, const Grid &G, hipStream_t s) {
     GDSJ_TRY(gd.scratch.ensure((size_t)d * 16 + 256 + (size_t)d * GS * 8));
     gd.xrs = (d + 15) / 16 * 16;
     GDSJ_TRY(gd.xr.ensure((size_t)n * gd.xrs * sizeof(T) + 256));
-    {
-        const uint64_t tot = (uint64_t)n * gd.xrs;
-        k_rows<T><<<(uint32_t)((tot + 255) / 256), 256, 0, s>>>(G.xs.as<T>(), G.ld, n, d, gd.xrs, gd.xr.as<T>());
+    if (n) {
+        if ((size_t)gd.xrs * 65 * sizeof(T) <= 48 * 1024)
+            k_rows<T, 64><<<ceil_div(n, 64), 256, (size_t)gd.xrs * 65 * sizeof(T), s>>>(G.xs.as<T>(), G.ld, n, d,
+                                                                                       gd.xrs, gd.xr.as<T>());
+        else
+            k_rows<T, 32><<<ceil_div(n, 32), 256, (size_t)gd.xrs * 33 * sizeof(T), s>>>(G.xs.as<T>(), G.ld, n, d,
+                                                                                       gd.xrs, gd.xr.as<T>());
+        GDSJ_HIP(hipGetLastError());
     }
     double *mean = gd.scratch.as<double>();
     double *maxdev = mean + d;
