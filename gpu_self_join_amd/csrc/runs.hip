@@ -62,7 +62,7 @@ static void combo_table(Grid &G, uint32_t *na, uint32_t *nb, uint32_t *split) {
 
 // R: combinations per lane and loop step (their lookups overlap)
 template <typename T, int R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R > 1 ? 4 : 8, 8))) void k_cell_runs(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R > 1 ? 6 : 8, 8))) void k_cell_runs(
     const T *__restrict__ xs, uint64_t ld, GridArgs ga, const uint32_t *__restrict__ cellStart,
     const uint32_t *__restrict__ cellCoord, const uint64_t *__restrict__ cellKey,
     const uint64_t *__restrict__ hkeys, const uint32_t *__restrict__ hvals, const uint4 *__restrict__ tab,
