@@ -524,15 +524,24 @@ int estimator_capacity_sampling(const Grid &G, const void *joinArgs, int mode, u
 }
 
 // ----------------------------------------------------------------- error --
+constexpr uint32_t EST_LANES = 8;
+
 __global__ void k_est_error(const float *__restrict__ predLog2, const uint4 *__restrict__ tiles, uint32_t nt,
                             const uint32_t *__restrict__ ids, const uint32_t *__restrict__ counts,
                             double *__restrict__ sumRel, unsigned long long *__restrict__ maxRelBits) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    double rel = 0.0;
+    // EST_LANES lanes per tile (aligned groups of a wave) over its queries
+    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / EST_LANES;
+    const uint32_t sub = threadIdx.x % EST_LANES;
+    const uint32_t t = g < nt ? g : nt;
+    unsigned long long y = 0;
     if (t < nt) {
         const uint4 tl = tiles[t];
-        uint64_t y = 0;
-        for (uint32_t q = tl.y; q < tl.z; q++) y += counts[ids[q]];
+        for (uint32_t q = tl.y + sub; q < tl.z; q += EST_LANES) y += counts[ids[q]];
+    }
+#pragma unroll
+    for (uint32_t off = EST_LANES / 2; off > 0; off >>= 1) y += __shfl_xor(y, off, GDSJ_WAVE);
+    double rel = 0.0;
+    if (t < nt && sub == 0) {
         const double p = fmax(exp2((double)predLog2[t]) - 1.0, 0.0);
         rel = fabs(p - (double)y) / fmax((double)y, 1.0);
     }
@@ -553,8 +562,8 @@ int estimator_error(const Grid &G, const float *predLog2, const uint32_t *counts
     GDSJ_HIP(hipMemsetAsync(scratch.p, 0, 16, s));
     double *sumRel = scratch.as<double>();
     unsigned long long *mx = (unsigned long long *)(sumRel + 1);
-    k_est_error<<<ceil_div(nt, RED_BLOCK), RED_BLOCK, 0, s>>>(predLog2, G.tiles.as<uint4>(), nt, G.ids.as<uint32_t>(), counts,
-                                                  sumRel, mx);
+    k_est_error<<<ceil_div(nt, RED_BLOCK / EST_LANES), RED_BLOCK, 0, s>>>(predLog2, G.tiles.as<uint4>(), nt,
+                                                                        G.ids.as<uint32_t>(), counts, sumRel, mx);
     double h[2] = {0, 0};
     GDSJ_HIP(hipMemcpyAsync(h, scratch.p, 16, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipStreamSynchronize(s));
