@@ -49,6 +49,7 @@ struct gdsj_handle {
     // slotted single pass (estimator-sized per-tile result slots)
     DevBuf slotCap, slotOff, slotUsed, slotOvf, slotIds, slotQ, scalars, estPred, estScratch, subTiles;
     DevBuf winTiles, winCnt, winOff;  // window splits of the overflowed Gram tiles
+    DevBuf sortWork, sampleScratch;   // row sort and sampling-estimator scratch
     Estimator est;
     GramData gram;
     DevBuf gramStats;
@@ -162,7 +163,7 @@ int gdsj_destroy(gdsj_handle *h) {
     DevBuf *b[] = {&h->counts,  &h->offsets,  &h->ids,     &h->data,    &h->scanTmp, &h->status,
                    &h->work,    &h->slotCap,  &h->slotOff, &h->slotUsed, &h->slotOvf, &h->slotIds,
                    &h->slotQ,   &h->scalars,  &h->estPred, &h->estScratch, &h->subTiles,
-                   &h->winTiles, &h->winCnt,  &h->winOff};
+                   &h->winTiles, &h->winCnt,  &h->winOff, &h->sortWork, &h->sampleScratch};
     h->est.release();
     h->gram.release();
     h->gramStats.release();
@@ -183,6 +184,10 @@ static int resolve_params(const gdsj_params *in, uint32_t dims, gdsj_params *out
     }
     if (out->ilp > 64) {
         set_error("ILP=%u exceeds 64", out->ilp);
+        return GDSJ_EINVAL;
+    }
+    if (out->ilp_fold > 1) {
+        set_error("ilp_fold=%u: 0 or 1", out->ilp_fold);
         return GDSJ_EINVAL;
     }
     if (out->stamp > 2) {
@@ -333,7 +338,7 @@ static int fill_finish(gdsj_handle *h, uint32_t *d_ids) {
     hipStream_t s = h->stream;
     if (h->p.sort_neighbors) {
         GDSJ_HIP(hipEventRecord(h->ev[8], s));
-        GDSJ_TRY(sort_rows(h->cur_offsets, h->G.n, d_ids, s));
+        GDSJ_TRY(sort_rows(h->cur_offsets, h->G.n, d_ids, s, h->sortWork));
         GDSJ_HIP(hipEventRecord(h->ev[6], s));
     }
     uint32_t flag = 0;
@@ -381,7 +386,7 @@ static int slotted_pass(gdsj_handle *h, JoinArgs &a, bool *used) {
         GDSJ_HIP(hipMemcpyAsync(h->estPred.p, h->est.pred.p, (size_t)nt * 4, hipMemcpyDeviceToDevice, s));
     } else {
         GDSJ_TRY(estimator_capacity_sampling(G, &a, h->mode, a.counts, h->slotCap.as<uint32_t>(), &predicted,
-                                             h->estPred.as<float>(), s));
+                                             h->estPred.as<float>(), s, h->sampleScratch));
     }
     GDSJ_TRY(h->scanTmp.ensure(scan_tmp_bytes((uint64_t)nt + 1) + scan_tmp_bytes((uint64_t)G.n + 1) + 256));
     GDSJ_TRY(exclusive_scan_u32_u64(h->slotCap.as<uint32_t>(), h->slotOff.as<uint64_t>(), nt, true,
@@ -433,7 +438,7 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
         h->thr = gdsj_eps_threshold_f64(epsilon);
     }
     // ILP > 0 splits the sum (kernel.cu:312-342); its fold depends on SHORTCIRCUIT
-    h->mode = h->p.ilp == 0 ? ACC_SEQ : (h->p.shortcircuit ? ACC_ILP_SC : ACC_ILP_NOSC);
+    h->mode = acc_mode(h->p.ilp, h->p.shortcircuit, h->p.ilp_fold);
     memset(&h->st, 0, sizeof(h->st));
     GDSJ_TRY(h->status.ensure(256));
     GDSJ_HIP(hipEventRecord(h->ev[0], s));
@@ -645,8 +650,8 @@ int gdsj_bruteforce_device(gdsj_handle *h, const void *d_data, uint32_t n, uint3
     }
     gdsj_params p;
     if (pin) p = *pin; else gdsj_params_default(&p);
-    if (p.ilp > 64) {
-        set_error("ILP=%u exceeds 64", p.ilp);
+    if (p.ilp > 64 || p.ilp_fold > 1) {
+        set_error("ILP=%u exceeds 64 or ilp_fold=%u is not 0 or 1", p.ilp, p.ilp_fold);
         return GDSJ_EINVAL;
     }
     double thr;
@@ -660,7 +665,7 @@ int gdsj_bruteforce_device(gdsj_handle *h, const void *d_data, uint32_t n, uint3
     } else {
         thr = gdsj_eps_threshold_f64(epsilon);
     }
-    const int mode = p.ilp == 0 ? ACC_SEQ : (p.shortcircuit ? ACC_ILP_SC : ACC_ILP_NOSC);
+    const int mode = acc_mode(p.ilp, p.shortcircuit, p.ilp_fold);
     GDSJ_HIP(hipSetDevice(h->device));
     h->have_counts = h->have_table = false;
     hipStream_t s = stream ? (hipStream_t)stream : h->own;

@@ -108,8 +108,6 @@ struct gdsj_batch {
 
 namespace {
 
-int mode_of(uint32_t ilp, uint32_t sc) { return ilp == 0 ? ACC_SEQ : (sc ? ACC_ILP_SC : ACC_ILP_NOSC); }
-
 void drop_graph(gdsj_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -251,7 +249,7 @@ BatchLaunch launch_of(gdsj_batch *b, int q, bool lpt) {
 
 int pass(gdsj_batch *b, const BatchLaunch &L, int ps, hipStream_t s, uint32_t *blocks, bool launch = true) {
     const JoinArgs *inst = b->dArgs.as<JoinArgs>();
-    const int mode = mode_of(b->tune.ilp, b->p.shortcircuit);
+    const int mode = acc_mode(b->tune.ilp, b->p.shortcircuit, b->p.ilp_fold);
     const uint32_t threads = b->tune.waves_per_block * GDSJ_WAVE;
     if (b->dtype == GDSJ_DTYPE_F32)
         return batch_pass_f32(inst, L, mode, b->tune.ilp, b->tune.reg_budget, ps, threads, blocks, s, launch);
@@ -380,6 +378,10 @@ int gdsj_batch_create(int device, const gdsj_instance *inst, uint32_t ninst, int
     const bool ilp_auto = p.ilp == GDSJ_ILP_AUTO;
     if (!ilp_auto && p.ilp > 64) {
         set_error("ILP=%u exceeds 64", p.ilp);
+        return GDSJ_EINVAL;
+    }
+    if (p.ilp_fold > 1) {
+        set_error("ilp_fold=%u: 0 or 1", p.ilp_fold);
         return GDSJ_EINVAL;
     }
     int ndev = 0;

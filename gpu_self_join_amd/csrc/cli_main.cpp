@@ -10,7 +10,7 @@
 // reference fixes dims, NUMINDEXEDDIM, DTYPE and the optimisation switches
 // at compile time (params.h); here they are options:
 //   --dtype float|double (default float)  --k K (default min(dim, 6))
-//   --ilp I  --shortcircuit 0|1  --reorder 0|1  --stamp 0|1  --linearstamp 0|1
+//   --ilp I  --shortcircuit 0|1  --ilp-fold 0|1  --reorder 0|1  --stamp 0|1  --linearstamp 0|1
 //   --queryreorder 0|1
 //   --estimator exact|learned|sample  --filter plain|gram|gram-bf16|auto  --device D
 //   --print   print every point's sorted neighbor list (PRINTNEIGHBORTABLE
@@ -36,7 +36,7 @@ static int usage() {
     fprintf(stderr,
             "usage: gdsjoin_main <dataset file> <epsilon> <dims> <mode> [options]\n"
             "  mode 3: GPU grid self-join; mode 1: all-pairs GPU join (total only)\n"
-            "  options: --dtype float|double --k K --ilp I --shortcircuit 0|1 --reorder 0|1\n"
+            "  options: --dtype float|double --k K --ilp I --shortcircuit 0|1 --ilp-fold 0|1 --reorder 0|1\n"
             "           --stamp 0|1 --linearstamp 0|1 --queryreorder 0|1 --estimator exact|learned|sample\n"
             "           --filter plain|gram|gram-bf16|auto --device D --print --table-file\n"
             "           --outlier-scores\n");
@@ -131,6 +131,7 @@ int main(int argc, char **argv) {
         } else if (!strcmp(a, "--k")) p.k = (uint32_t)atoi(v);
         else if (!strcmp(a, "--ilp")) p.ilp = (uint32_t)atoi(v);
         else if (!strcmp(a, "--shortcircuit")) p.shortcircuit = (uint32_t)atoi(v);
+        else if (!strcmp(a, "--ilp-fold")) p.ilp_fold = (uint32_t)atoi(v);
         else if (!strcmp(a, "--reorder")) p.reorder = (uint32_t)atoi(v);
         else if (!strcmp(a, "--stamp")) p.stamp = atoi(v) ? 1u : (p.stamp == 2 ? 2u : 0u);
         else if (!strcmp(a, "--linearstamp")) p.stamp = atoi(v) ? 2u : (p.stamp == 1 ? 1u : 0u);

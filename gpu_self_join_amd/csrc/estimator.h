@@ -62,8 +62,10 @@ int estimator_train_rows(Estimator &E, const float *feat, const float *target, u
 int estimator_capacity_learned(Estimator &E, const Grid &G, uint32_t *capOut, uint64_t *predicted,
                                hipStream_t s);
 // The reference's sampling estimator restated, spread uniformly per query.
+// scratch: n + 4 * (number of sampled queries) words, kept by the caller.
 int estimator_capacity_sampling(const Grid &G, const void *joinArgs, int mode, uint32_t *countsScratch,
-                                uint32_t *capOut, uint64_t *predicted, float *predLog2Out, hipStream_t s);
+                                uint32_t *capOut, uint64_t *predicted, float *predLog2Out, hipStream_t s,
+                                DevBuf &scratch);
 
 // mean and max of |pred_t - y_t| / max(y_t, 1) over tiles (pred from capacities' model output)
 int estimator_error(const Grid &G, const float *predLog2, const uint32_t *counts, double *meanRel,

@@ -482,12 +482,12 @@ __global__ void k_sample_sum(const uint32_t *__restrict__ counts, uint32_t ns, u
 }
 
 int estimator_capacity_sampling(const Grid &G, const void *joinArgs, int mode, uint32_t *countsScratch,
-                                uint32_t *capOut, uint64_t *predicted, float *predLog2Out, hipStream_t s) {
+                                uint32_t *capOut, uint64_t *predicted, float *predLog2Out, hipStream_t s,
+                                DevBuf &scratch) {
     const uint32_t n = G.n, nt = G.ntiles;
     const double sampleRate = 0.015;             // SAMPLERATE, params.h:93
     const uint32_t stride = (uint32_t)(1.0 / sampleRate);  // offsetRate = 66, GPU.cu:86
     const uint32_t ns = (uint32_t)(n * sampleRate);        // N_batchEst, GPU.cu:100
-    static thread_local DevBuf scratch;
     GDSJ_TRY(scratch.ensure((size_t)n * 4 + (size_t)ns * 16 + 256));
     uint32_t *inv = scratch.as<uint32_t>();
     uint4 *stiles = (uint4 *)(((uintptr_t)(inv + n) + 255) & ~(uintptr_t)255);
@@ -524,32 +524,47 @@ int estimator_capacity_sampling(const Grid &G, const void *joinArgs, int mode, u
 }
 
 // ----------------------------------------------------------------- error --
-constexpr uint32_t EST_LANES = 8;
+// one thread per tile: the tile's hits (a loop over its queries, 3 on
+// average at 6 dims) against the model's prediction, in float; a block's
+// sum and maximum go to the totals with one atomic each
+constexpr uint32_t EST_BLOCK = 256;
 
-__global__ void k_est_error(const float *__restrict__ predLog2, const uint4 *__restrict__ tiles, uint32_t nt,
-                            const uint32_t *__restrict__ ids, const uint32_t *__restrict__ counts,
-                            double *__restrict__ sumRel, unsigned long long *__restrict__ maxRelBits) {
-    // EST_LANES lanes per tile (aligned groups of a wave) over its queries
-    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / EST_LANES;
-    const uint32_t sub = threadIdx.x % EST_LANES;
-    const uint32_t t = g < nt ? g : nt;
-    unsigned long long y = 0;
+__global__ __launch_bounds__(EST_BLOCK) void k_est_error(const float *__restrict__ predLog2,
+                                                         const uint4 *__restrict__ tiles, uint32_t nt,
+                                                         const uint32_t *__restrict__ ids,
+                                                         const uint32_t *__restrict__ counts,
+                                                         double *__restrict__ sumRel,
+                                                         unsigned int *__restrict__ maxRelBits) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    float rel = 0.0f;
     if (t < nt) {
         const uint4 tl = tiles[t];
-        for (uint32_t q = tl.y + sub; q < tl.z; q += EST_LANES) y += counts[ids[q]];
+        uint32_t y = 0;
+        for (uint32_t q = tl.y; q < tl.z; q++) y += counts[ids[q]];
+        const float p = fmaxf(exp2f(predLog2[t]) - 1.0f, 0.0f);
+        rel = fabsf(p - (float)y) / fmaxf((float)y, 1.0f);
     }
-#pragma unroll
-    for (uint32_t off = EST_LANES / 2; off > 0; off >>= 1) y += __shfl_xor(y, off, GDSJ_WAVE);
-    double rel = 0.0;
-    if (t < nt && sub == 0) {
-        const double p = fmax(exp2((double)predLog2[t]) - 1.0, 0.0);
-        rel = fabs(p - (double)y) / fmax((double)y, 1.0);
+    float sm = rel, mx = rel;
+    for (int off = 32; off > 0; off >>= 1) {
+        sm += __shfl_xor(sm, off, GDSJ_WAVE);
+        mx = fmaxf(mx, __shfl_xor(mx, off, GDSJ_WAVE));
     }
-    const double sm = block_reduce<double, false>(rel);
-    const double mx = block_reduce<double, true>(rel);
+    __shared__ float ps[EST_BLOCK / GDSJ_WAVE], pm[EST_BLOCK / GDSJ_WAVE];
+    const uint32_t w = threadIdx.x / GDSJ_WAVE;
+    if (lane_id() == 0) {
+        ps[w] = sm;
+        pm[w] = mx;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        atomicAdd(sumRel, sm);
-        atomicMax(maxRelBits, (unsigned long long)__double_as_longlong(mx));
+        double a = 0.0;
+        float m = 0.0f;
+        for (uint32_t i = 0; i < EST_BLOCK / GDSJ_WAVE; i++) {
+            a += ps[i];
+            m = fmaxf(m, pm[i]);
+        }
+        atomicAdd(sumRel, a);
+        atomicMax(maxRelBits, __float_as_uint(m));  // non-negative floats order as their bits
     }
 }
 
@@ -561,17 +576,17 @@ int estimator_error(const Grid &G, const float *predLog2, const uint32_t *counts
     GDSJ_TRY(scratch.ensure(64));
     GDSJ_HIP(hipMemsetAsync(scratch.p, 0, 16, s));
     double *sumRel = scratch.as<double>();
-    unsigned long long *mx = (unsigned long long *)(sumRel + 1);
-    k_est_error<<<ceil_div(nt, RED_BLOCK / EST_LANES), RED_BLOCK, 0, s>>>(predLog2, G.tiles.as<uint4>(), nt,
-                                                                        G.ids.as<uint32_t>(), counts, sumRel, mx);
+    unsigned int *mx = (unsigned int *)(sumRel + 1);
+    k_est_error<<<ceil_div(nt, EST_BLOCK), EST_BLOCK, 0, s>>>(predLog2, G.tiles.as<uint4>(), nt,
+                                                              G.ids.as<uint32_t>(), counts, sumRel, mx);
     double h[2] = {0, 0};
     GDSJ_HIP(hipMemcpyAsync(h, scratch.p, 16, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipStreamSynchronize(s));
     *meanRel = h[0] / nt;
-    uint64_t bits;
-    memcpy(&bits, &h[1], 8);
-    double m;
-    memcpy(&m, &bits, 8);
+    uint32_t bits;
+    memcpy(&bits, &h[1], 4);
+    float m;
+    memcpy(&m, &bits, 4);
     *maxRel = m;
     return GDSJ_OK;
 }

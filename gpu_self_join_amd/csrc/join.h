@@ -12,6 +12,14 @@ enum AccMode : int {
     ACC_ILP_NOSC = 2,  // ILP > 0, SHORTCIRCUIT == 0: ILP partials, fold at end
 };
 
+// The accumulation a join runs for ILP, SHORTCIRCUIT and the fold reading
+// (gdsj_params.ilp_fold): with every dim fused into the running sum, ILP > 0
+// with SHORTCIRCUIT is the sequential sum of ILP = 0.
+inline int acc_mode(uint32_t ilp, uint32_t sc, uint32_t fold) {
+    if (ilp == 0 || (sc && fold)) return ACC_SEQ;
+    return sc ? ACC_ILP_SC : ACC_ILP_NOSC;
+}
+
 enum JoinPass : int { PASS_COUNT = 0, PASS_FILL = 1, PASS_SLOT = 2 };
 
 // Everything the join kernels read besides the coordinates and ids

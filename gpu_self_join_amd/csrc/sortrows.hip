@@ -92,9 +92,8 @@ __global__ __launch_bounds__(256) void k_sort_rows_big(const uint64_t *__restric
     }
 }
 
-int sort_rows(const uint64_t *offs, uint32_t n, uint32_t *ids, hipStream_t s) {
+int sort_rows(const uint64_t *offs, uint32_t n, uint32_t *ids, hipStream_t s, DevBuf &work) {
     if (n == 0) return GDSJ_OK;
-    static thread_local DevBuf work;
     GDSJ_TRY(work.ensure((size_t)n * 4 + 256));
     uint32_t *nbig = work.as<uint32_t>();
     uint32_t *big = nbig + 64;

@@ -111,18 +111,23 @@ WORKLOADS = {
 }
 
 
-def generate(name, n=None):
-    """Data of a workload (optionally only its first n points, same stream)."""
+WORKLOAD_SEEDS = {"uniform2d": 1, "gmm6d": 2, "aniso32d": 3, "lowid64d": 4}
+
+
+def generate(name, n=None, seed=None):
+    """Data of a workload (optionally only its first n points, same stream);
+    seed replaces the workload's own (another draw of the same family)."""
     w = WORKLOADS[name]
     nn = w["n"]
+    sd = WORKLOAD_SEEDS[name] if seed is None else seed
     if name == "uniform2d":
-        x = uniform(nn, 2, 0.0, 100.0, seed=1)
+        x = uniform(nn, 2, 0.0, 100.0, seed=sd)
     elif name == "gmm6d":
-        x = gmm(nn, 6, 32, 2.0, 0.05, seed=2)
+        x = gmm(nn, 6, 32, 2.0, 0.05, seed=sd)
     elif name == "aniso32d":
-        x = aniso(nn, 32, 16, 4.0, 0.9, seed=3)
+        x = aniso(nn, 32, 16, 4.0, 0.9, seed=sd)
     elif name == "lowid64d":
-        x = lowid(nn, 64, 8, 10.0, 0.05, seed=4)
+        x = lowid(nn, 64, 8, 10.0, 0.05, seed=sd)
     else:
         raise KeyError(name)
     return x if n is None else x[:n].copy()

@@ -83,6 +83,13 @@ typedef struct gdsj_params {
                                instead of recomputing distances: 0 auto (on from 4 dims
                                when they take at most 8 GiB), 1 on (error if they do not
                                fit), 2 off (speed only) */
+    uint32_t ilp_fold;      /* ILP > 0 with SHORTCIRCUIT = 1: which contraction of the fold
+                               "runningDist[0] += runningDist[j]" (kernel.cu:320-334) the sums
+                               follow.  0: each group's first dim fused into the running sum,
+                               the others rounded (t*t) and added (default); 1: every dim fused
+                               into the running sum, which is the arithmetic of ILP = 0.  The
+                               two readings differ only on pairs whose sums straddle the
+                               threshold by a few ulps */
 } gdsj_params;
 
 /* Timing and diagnostics of the last join on a handle. */

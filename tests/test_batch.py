@@ -171,19 +171,24 @@ def test_batch_rejects_bad_arguments(gpu):
         BatchJoin([x], [0.5], estimator=_lib.EST_SAMPLE)
 
 
+@pytest.mark.timeout(900)
 def test_batched_workload_equals_single_joins(gpu):
-    """The 256-instance workload: every instance's rows equal a single join's."""
+    """The 256-instance workload: every instance's rows equal a single join's,
+    and every instance's counts and per-point neighbor-id hashes equal the
+    oracle's (the oracle's ILP fold ambiguity count is printed)."""
     import torch
 
     from gpu_self_join_amd import BatchJoin, SelfJoin, datagen
     m = datagen.BATCHED_COUNT
-    xs, eps = [], []
+    xs, eps, host = [], [], []
     for i in range(m):
         x, _ = datagen.batched_instance(i)
+        host.append(x)
         xs.append(torch.from_numpy(x).cuda())
         eps.append(datagen.batched_eps(i))
     bj = BatchJoin(xs, eps, estimator=_lib.EST_LEARNED)
     sj = SelfJoin(0)
+    amb = 0
     try:
         bj.run()
         r = bj.run()
@@ -193,6 +198,14 @@ def test_batched_workload_equals_single_joins(gpu):
             s = sj.join(xs[i], eps[i], filter=_lib.FILTER_PLAIN)
             assert torch.equal(c, s.counts), i
             assert torch.equal(ids, s.ids), i
+            x = host[i]
+            ref = O.join(x, eps[i], k=min(x.shape[1], 6), ilp=8, shortcircuit=1, reorder=1, with_ids=False,
+                         nthreads=16)
+            cn = c.cpu().numpy().view(np.uint32)
+            assert np.array_equal(cn, ref["counts"]), i
+            assert np.array_equal(O.csr_hashes(cn, ids.cpu().numpy().view(np.uint32)), ref["hashes"]), i
+            amb += int(ref["stats"][3])
+        print("batched: all %d instances equal the oracle; ILP fold ambiguous pairs %d" % (m, amb), flush=True)
     finally:
         bj.close()
         sj.close()

@@ -166,3 +166,36 @@ def test_learned_estimator_beats_sampling_on_clustered_data(gpu):
         assert sl["est_mean_rel_err"] < ss["est_mean_rel_err"]
     finally:
         sj.close()
+
+
+@pytest.mark.gpu
+def test_learned_estimator_gain_holds_out_of_sample(gpu):
+    """The network trained on one dataset, then frozen (train = 2), sizes
+    joins it has not seen: another seed of the same family and the training
+    data at 0.9 and 1.1 times epsilon.  On each it beats the sampling
+    estimator in mean relative error per tile and in overflowed tiles."""
+    from gpu_self_join_amd import SelfJoin
+    sj = SelfJoin(0)
+    try:
+        kw = dict(k=6, ilp=8, shortcircuit=1, reorder=1)
+        x = _gmm(200000, 6, 10, ncl=20, sigma=1.5)
+        eps = 0.9
+        _join(sj, x, eps, estimator=_lib.EST_LEARNED, train=0, **kw)  # exact passes + training
+        held = [("seed 12", _gmm(200000, 6, 12, ncl=20, sigma=1.5), eps),
+                ("eps x 0.9", x, float(np.float32(eps * 0.9))),
+                ("eps x 1.1", x, float(np.float32(eps * 1.1)))]
+        for name, xh, eh in held:
+            _, _, rl = _join(sj, xh, eh, estimator=_lib.EST_LEARNED, train=2, **kw)
+            _, _, rs = _join(sj, xh, eh, estimator=_lib.EST_SAMPLE, **kw)
+            sl, ss = rl.stats, rs.stats
+            assert sl["est_tiles"] > 0 and ss["est_tiles"] > 0
+            ovl = sl["est_overflow_tiles"] / sl["est_tiles"]
+            ovs = ss["est_overflow_tiles"] / ss["est_tiles"]
+            print("%s: learned overflow %.4f mean rel err %.3f capacity/pairs %.2f | sampling overflow %.4f "
+                  "mean rel err %.3f capacity/pairs %.2f" % (
+                      name, ovl, sl["est_mean_rel_err"], sl["est_capacity"] / rl.total, ovs,
+                      ss["est_mean_rel_err"], ss["est_capacity"] / rs.total), flush=True)
+            assert sl["est_mean_rel_err"] < ss["est_mean_rel_err"], name
+            assert ovl < ovs, name
+    finally:
+        sj.close()

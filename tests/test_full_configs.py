@@ -6,8 +6,14 @@ through size-independent summaries:
   the neighbor ids), against the oracle's, for the 2-D and 6-D workloads
   (the oracle runs the whole join);
 - for the 32-D and 64-D workloads, whose full oracle join would take hours
-  on the host, the oracle's counts and hashes of a seeded query sample, plus
-  byte equality of the MFMA-filtered table with the plain table at full size;
+  on the host, the oracle's counts and hashes of a seeded 1 % query sample
+  that holds the points of the most occupied cells (the heaviest
+  neighborhoods), plus byte equality of the MFMA-filtered table with the
+  plain table at full size;
+- the oracle's count of pairs on which the two readings of the ILP fold
+  (kernel.cu:320-334: fused or rounded products added) disagree, printed per
+  workload; for gmm6d the GPU's fused reading (ilp_fold = 1) is also checked
+  against the oracle's table for that reading;
 - CSR invariants: offsets are the prefix sum of counts, every point is its
   own neighbor, the relation is symmetric (sum of counts equals the number
   of (i, j) with i in row j, checked through hashes of transposed pairs on a
@@ -18,9 +24,10 @@ import pytest
 
 from gpu_self_join_amd import _lib, datagen
 from oracle import oracle as O
-from tests.helpers import sorted_rows
+from tests.helpers import dense_sample, sorted_rows
 
-pytestmark = pytest.mark.gpu
+# full-size oracle runs take minutes on the host
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
 
 
 def _run(sj, pts, eps, **kw):
@@ -69,11 +76,23 @@ def test_full_size_table_equals_oracle(gpu, name):
         counts, offsets, ids, r = _run(sj, pts, eps, **kw)
         print(name, "pairs", r.total, flush=True)
         _check_csr(counts, offsets, ids)
+        print(name, "oracle join (default fold reading)", flush=True)
         ref = O.join(x, eps, k=p["k"], ilp=p["ilp"], shortcircuit=p["shortcircuit"], reorder=p["reorder"],
                      with_ids=False, nthreads=16)
+        print(name, "ILP fold ambiguous pairs", int(ref["stats"][3]), flush=True)
         assert np.array_equal(counts, ref["counts"])
         assert np.array_equal(O.csr_hashes(counts, ids), ref["hashes"])
         _symmetric_sample(counts, offsets, ids)
+        if p["ilp"] > 0 and p["shortcircuit"]:
+            # the fused reading of the fold, on the GPU and in the oracle
+            cf, of, idf, rf = _run(sj, pts, eps, ilp_fold=1, **kw)
+            print(name, "oracle join (fused fold reading)", flush=True)
+            reff = O.join(x, eps, k=p["k"], ilp=p["ilp"], shortcircuit=p["shortcircuit"], reorder=p["reorder"],
+                          fold_fused=1, with_ids=False, nthreads=16)
+            assert np.array_equal(cf, reff["counts"])
+            assert np.array_equal(O.csr_hashes(cf, idf), reff["hashes"])
+            ndiff = int(np.abs(cf.astype(np.int64) - counts.astype(np.int64)).sum())
+            assert ndiff <= int(ref["stats"][3])
         # the learned single pass writes the same table
         sl = SelfJoin(0)
         try:
@@ -109,10 +128,12 @@ def test_full_size_filtered_table_equals_plain_and_oracle_sample(gpu, name):
         _check_csr(cg, og, ig)
         cp, op, ip, rp = _run(sj, pts, eps, filter=_lib.FILTER_PLAIN, **kw)
         assert np.array_equal(cg, cp) and np.array_equal(ig, ip)
-        rng = np.random.default_rng(77)
-        q = np.sort(rng.choice(len(x), 1500, replace=False)).astype(np.uint32)
+        q = dense_sample(x, eps, p, frac=0.01)
+        print(name, "oracle sample join", flush=True)
         ref = O.join_subset(x, eps, p["k"], q, ilp=p["ilp"], shortcircuit=p["shortcircuit"], reorder=p["reorder"],
                             nthreads=16)
+        print(name, "oracle sample", len(q), "queries,", int(ref["stats"][1]), "candidates; ILP fold ambiguous pairs",
+              int(ref["stats"][3]), flush=True)
         assert np.array_equal(cg[q], ref["counts"])
         h = O.csr_hashes(cg, ig)
         assert np.array_equal(h[q], ref["hashes"])

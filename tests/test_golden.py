@@ -49,3 +49,14 @@ def test_gpu_matches_fixture(sj, path):
         i = r.ids.cpu().numpy().view(np.uint32)
         assert np.array_equal(c, counts), m
         assert np.array_equal(sorted_rows(c, i), ids), m
+    if p["ilp"] > 0 and p["shortcircuit"]:
+        # the other reading of the ILP fold (every dim fused) against the
+        # oracle's table for that reading
+        ref = O.join(x, eps, fold_fused=1, **p)
+        print(os.path.basename(path), "ambiguous pairs", int(ref["stats"][3]), flush=True)
+        for m in modes:
+            r = sj.join(pts, eps, ilp_fold=1, **p, **m)
+            c = r.counts.cpu().numpy().view(np.uint32)
+            i = r.ids.cpu().numpy().view(np.uint32)
+            assert np.array_equal(c, ref["counts"]), m
+            assert np.array_equal(sorted_rows(c, i), sorted_rows(ref["counts"], ref["ids"])), m

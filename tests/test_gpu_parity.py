@@ -57,6 +57,23 @@ def test_readme_through_gdsjoin_dropin(gpu):
     assert_csr_equal(counts, table, ref["counts"], ref["ids"])
 
 
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_ilp_fold_readings_on_straddling_pairs(sj, dt):
+    """Data with pairs on which the two readings of the ILP fold disagree:
+    ilp_fold = 0 gives the oracle's default table, ilp_fold = 1 the fused
+    one, and the two differ."""
+    from tests.helpers import fold_straddle_data
+    x, eps, found = fold_straddle_data(dt)
+    assert found >= 8
+    for fold in (0, 1):
+        c, ids, r = _gpu_join(sj, x, eps, k=6, ilp=8, shortcircuit=1, reorder=0, ilp_fold=fold)
+        ref = O.join(x, eps, k=6, ilp=8, shortcircuit=1, reorder=0, fold_fused=fold)
+        assert_csr_equal(c, ids, ref["counts"], ref["ids"])
+        if fold == 0:
+            c0 = c
+    assert not np.array_equal(c0, c)
+
+
 CASES = [
     # (n, d, k, dtype, ilp, sc, reorder, neighbors)
     (3000, 2, 2, np.float32, 0, 0, 0, 30),

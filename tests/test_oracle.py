@@ -192,3 +192,49 @@ def test_cpu_baselines_rtree_equals_bruteforce():
         # the epsilon boundary, which random data almost never hits
         g = O.join(x, e, k=min(d, 6), with_ids=False)["counts"][q]
         assert int(np.abs(g.astype(np.int64) - cb).sum()) <= 2
+
+
+# ---- the two readings of the ILP fold ----------------------------------------
+def _pairs(counts, ids):
+    rows = np.repeat(np.arange(len(counts), dtype=np.uint64), counts.astype(np.int64))
+    return set((rows << np.uint64(32) | ids.astype(np.uint64)).tolist())
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_fold_ambiguity_count_equals_table_difference(dt):
+    """stats[3] (pairs on which the fused and the unfused fold disagree) is
+    the size of the symmetric difference of the two readings' tables, on
+    data built to contain such pairs."""
+    from tests.helpers import fold_straddle_data
+    x, eps, found = fold_straddle_data(dt)
+    assert found >= 8
+    r0 = O.join(x, eps, k=6, ilp=8, shortcircuit=1, reorder=0, fold_fused=0)
+    r1 = O.join(x, eps, k=6, ilp=8, shortcircuit=1, reorder=0, fold_fused=1)
+    diff = _pairs(r0["counts"], r0["ids"]) ^ _pairs(r1["counts"], r1["ids"])
+    assert len(diff) >= 2 * found
+    assert int(r0["stats"][3]) == len(diff) == int(r1["stats"][3])
+    # the fused reading is the ILP = 0 table
+    s = O.join(x, eps, k=6, ilp=0, reorder=0)
+    assert np.array_equal(s["counts"], r1["counts"]) and np.array_equal(s["ids"], r1["ids"])
+
+
+def test_fold_ambiguity_on_golden_fixtures():
+    """On the ILP > 0 / SHORTCIRCUIT fixtures the counter agrees with the
+    tables of both readings."""
+    import glob
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    seen = 0
+    for path in sorted(glob.glob(os.path.join(here, "grid_*.npz"))):
+        z = np.load(path)
+        k, ilp, sc, reorder = (int(v) for v in z["params"])
+        if ilp == 0 or sc == 0:
+            continue
+        x, eps = z["data"], float(z["eps"])
+        r0 = O.join(x, eps, k=k, ilp=ilp, shortcircuit=sc, reorder=reorder, fold_fused=0)
+        r1 = O.join(x, eps, k=k, ilp=ilp, shortcircuit=sc, reorder=reorder, fold_fused=1)
+        diff = _pairs(r0["counts"], r0["ids"]) ^ _pairs(r1["counts"], r1["ids"])
+        assert int(r0["stats"][3]) == len(diff), path
+        print(os.path.basename(path), "ambiguous pairs", len(diff))
+        seen += 1
+    assert seen >= 3
