@@ -71,6 +71,15 @@ int orc_max_threads(void) { return omp_get_max_threads(); }
 
 #define T float
 #define SFX(x) x##_f32
+/* The per-query loop is built twice, for hosts with and without the FMA
+ * instruction (picked at load time); fmaf / fma stay correctly rounded
+ * either way, the instruction only avoids a library call per term. */
+#if defined(__x86_64__)
+#define ORC_HOT __attribute__((target_clones("fma", "default")))
+#else
+#define ORC_HOT
+#endif
+
 #define FMA fmaf
 #define SQRT sqrtf
 #define CEIL ceilf

@@ -71,18 +71,55 @@ void SFX(orc_reorder_perm)(const T *data, uint32_t n, uint32_t d,
  * ILP == m, SHORTCIRCUIT == 0 (kernel.cu:320-340): r_j = fma(t,t,r_j)
  *   over all chunks, then r0 = ((r0 + r1) + r2) ... + r_{m-1}.
  * SHORTCIRCUIT's early exit cannot change the outcome (partial sums of
- * non-negative terms never decrease under round-to-nearest), so the full sum
- * is always formed here. */
-static inline T SFX(orc_dist2)(const T *p, const T *q, uint32_t d,
-                               uint32_t ilp, uint32_t sc, uint32_t fold_fused)
+ * non-negative terms never decrease under round-to-nearest).  orc_dist2
+ * forms the full sum; orc_dist2_bounded (the join's checker) stops once the
+ * running sum exceeds stop, which returns a value > stop and so cannot
+ * change a predicate decided below stop. */
+static inline T SFX(orc_dist2_bounded)(const T *p, const T *q, uint32_t d,
+                                       uint32_t ilp, uint32_t sc, uint32_t fold_fused, T stop)
 {
     if (ilp == 0) {
         T acc = 0;
         for (uint32_t l = 0; l < d; l++) {
             T t = q[l] - p[l];
             acc = FMA(t, t, acc);
+            if ((l & 7) == 7 && acc > stop) return acc;
         }
         return acc;
+    }
+    if (ilp == 8 && sc && !fold_fused) {
+        /* the same steps as the general loop below with the group size
+         * fixed (params.h's ILP): fma(t,t,+0) of a later dim of a group is
+         * the rounded product, and adding a missing dim's +0 leaves the
+         * non-negative sum unchanged */
+        T r0 = 0;
+        uint32_t l = 0;
+        for (; l + 8 <= d; l += 8) {
+            const T t0 = q[l] - p[l], t1 = q[l + 1] - p[l + 1], t2 = q[l + 2] - p[l + 2],
+                    t3 = q[l + 3] - p[l + 3], t4 = q[l + 4] - p[l + 4], t5 = q[l + 5] - p[l + 5],
+                    t6 = q[l + 6] - p[l + 6], t7 = q[l + 7] - p[l + 7];
+            const T s1 = t1 * t1, s2 = t2 * t2, s3 = t3 * t3, s4 = t4 * t4, s5 = t5 * t5, s6 = t6 * t6,
+                    s7 = t7 * t7;
+            r0 = FMA(t0, t0, r0);
+            r0 = r0 + s1;
+            r0 = r0 + s2;
+            r0 = r0 + s3;
+            r0 = r0 + s4;
+            r0 = r0 + s5;
+            r0 = r0 + s6;
+            r0 = r0 + s7;
+            if (r0 > stop) return r0;
+        }
+        if (l < d) {
+            const T t0 = q[l] - p[l];
+            r0 = FMA(t0, t0, r0);
+            for (uint32_t j = l + 1; j < d; j++) {
+                const T t = q[j] - p[j];
+                const T sq = t * t;
+                r0 = r0 + sq;
+            }
+        }
+        return r0;
     }
     T r[64];
     for (uint32_t j = 0; j < ilp; j++) r[j] = 0;
@@ -100,6 +137,7 @@ static inline T SFX(orc_dist2)(const T *p, const T *q, uint32_t d,
                     r[0] = r[0] + r[j];
                     r[j] = 0;
                 }
+            if (r[0] > stop) return r[0];
         }
         return r[0];
     }
@@ -110,6 +148,12 @@ static inline T SFX(orc_dist2)(const T *p, const T *q, uint32_t d,
         }
     for (uint32_t j = 1; j < ilp; j++) r[0] = r[0] + r[j];
     return r[0];
+}
+
+static inline T SFX(orc_dist2)(const T *p, const T *q, uint32_t d,
+                               uint32_t ilp, uint32_t sc, uint32_t fold_fused)
+{
+    return SFX(orc_dist2_bounded)(p, q, d, ilp, sc, fold_fused, (T)INFINITY);
 }
 
 /* exported form of the accumulation, for arithmetic tests */
@@ -126,6 +170,7 @@ typedef struct {
     uint32_t n, d, k;
     T eps;
     T *pts;              /* n x d, dimensions already permuted (AoS)      */
+    T *spts;             /* the same rows in cell order (order[])          */
     T minArr[ORC_MAXK];
     uint32_t nCells[ORC_MAXK];
     uint64_t totalCells;
@@ -147,7 +192,7 @@ static int SFX(orc_kv_cmp)(const void *a, const void *b)
 
 static void SFX(orc_grid_free)(SFX(orc_grid_t) *g)
 {
-    free(g->pts); free(g->cellOf); free(g->order); free(g->cellId); free(g->cellStart);
+    free(g->pts); free(g->spts); free(g->cellOf); free(g->order); free(g->cellId); free(g->cellStart);
     memset(g, 0, sizeof(*g));
 }
 
@@ -224,6 +269,12 @@ static int SFX(orc_grid_build)(SFX(orc_grid_t) *g, const T *data, uint32_t n,
     g->cellStart[nc] = n;
     g->ncell = nc;
     free(kv);
+    /* the points again in cell order: a cell's candidates are read as one
+     * contiguous block */
+    g->spts = (T *)malloc(sizeof(T) * (size_t)n * d + 1);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; i++)
+        memcpy(g->spts + (size_t)i * d, g->pts + (size_t)g->order[i] * d, sizeof(T) * d);
     return 0;
 }
 
@@ -293,7 +344,7 @@ static int SFX(orc_u32_cmp)(const void *a, const void *b)
 
 /* core: evaluates query point qi against every candidate of its adjacent
  * cells.  Returns the count; optionally appends ids and accumulates stats. */
-static uint32_t SFX(orc_query)(const SFX(orc_grid_t) *g, const int64_t *adj,
+ORC_HOT static uint32_t SFX(orc_query)(const SFX(orc_grid_t) *g, const int64_t *adj,
                                uint32_t nadj, uint32_t qi, const orc_params *p,
                                uint64_t *hash, uint32_t *ids, uint64_t *ncand,
                                uint64_t *nambig)
@@ -302,15 +353,21 @@ static uint32_t SFX(orc_query)(const SFX(orc_grid_t) *g, const int64_t *adj,
     uint32_t cnt = 0;
     uint64_t h = 0, cand = 0, amb = 0;
     int check_amb = (p->ilp > 0 && p->shortcircuit && nambig != NULL);
+    /* The two fold readings differ by a few ulps of the sum: a sum outside
+     * eps^2 * (1 -+ 2^-10) decides both alike, and a running sum past the
+     * upper end cannot come back below eps^2 (the sums stop there). */
+    const T e2 = g->eps * g->eps;
+    const T band = e2 * (T)(1.0 / 1024.0);
+    const T stop = e2 + band;
     for (uint32_t a = 0; a < nadj; a++) {
         uint32_t s = g->cellStart[adj[a]], e = g->cellStart[adj[a] + 1];
         cand += e - s;
         for (uint32_t t = s; t < e; t++) {
             uint32_t cj = g->order[t];
-            const T *pc = g->pts + (size_t)cj * g->d;
-            T acc = SFX(orc_dist2)(pq, pc, g->d, p->ilp, p->shortcircuit, p->fold_fused);
+            const T *pc = g->spts + (size_t)t * g->d;
+            T acc = SFX(orc_dist2_bounded)(pq, pc, g->d, p->ilp, p->shortcircuit, p->fold_fused, stop);
             int hit = SFX(orc_pred)(acc, g->eps);
-            if (check_amb) {
+            if (check_amb && acc >= e2 - band && acc <= stop) {
                 T acc2 = SFX(orc_dist2)(pq, pc, g->d, p->ilp, p->shortcircuit, !p->fold_fused);
                 if (SFX(orc_pred)(acc2, g->eps) != hit) amb++;
             }

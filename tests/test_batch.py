@@ -205,6 +205,8 @@ def test_batched_workload_equals_single_joins(gpu):
             assert np.array_equal(cn, ref["counts"]), i
             assert np.array_equal(O.csr_hashes(cn, ids.cpu().numpy().view(np.uint32)), ref["hashes"]), i
             amb += int(ref["stats"][3])
+            if i % 16 == 15:
+                print("batched: instances 0..%d equal the oracle" % i, flush=True)
         print("batched: all %d instances equal the oracle; ILP fold ambiguous pairs %d" % (m, amb), flush=True)
     finally:
         bj.close()
