@@ -45,7 +45,8 @@ class GdsjParams(C.Structure):
                 ("filter", C.c_uint32), ("estimator", C.c_uint32), ("sort_neighbors", C.c_uint32),
                 ("verbose", C.c_uint32), ("tile_queries", C.c_uint32), ("schedule", C.c_uint32),
                 ("prune", C.c_uint32), ("train", C.c_uint32), ("filter_stats", C.c_uint32),
-                ("hitmask", C.c_uint32), ("ilp_fold", C.c_uint32)]
+                ("hitmask", C.c_uint32), ("ilp_fold", C.c_uint32),
+                ("vfilter", C.c_uint32)]
 
 
 class GdsjStats(C.Structure):

@@ -186,6 +186,10 @@ static int resolve_params(const gdsj_params *in, uint32_t dims, gdsj_params *out
         set_error("ILP=%u exceeds 64", out->ilp);
         return GDSJ_EINVAL;
     }
+    if (out->vfilter > 2) {
+        set_error("vfilter=%u: 0 auto, 1 on, 2 off", out->vfilter);
+        return GDSJ_EINVAL;
+    }
     if (out->ilp_fold > 1) {
         set_error("ilp_fold=%u: 0 or 1", out->ilp_fold);
         return GDSJ_EINVAL;
@@ -232,6 +236,10 @@ static JoinArgs make_args(gdsj_handle *h) {
         a.fcounts = h->fcnt.as<uint32_t>();
         a.rcur = h->rcur.as<uint32_t>();
     }
+    // the Gram-form prefilter on request (auto leaves it off: on gmm6d's
+    // slotted pass it issues 30 % fewer VALU instructions but runs 15 %
+    // slower, its per-chunk band test and branches serialising the waves)
+    if (h->p.vfilter == 1 && gram_valu_band(h->G, h->thr, &a.gvLo, &a.gvMid, &a.gvHalf)) a.gv = 1;
     if (h->G.haveRuns) {
         a.runs = h->G.runs.as<uint2>();
         a.nruns = h->G.nruns.as<uint32_t>();

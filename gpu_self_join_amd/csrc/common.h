@@ -135,4 +135,29 @@ struct DevBuf {
     }
 };
 
+// pinned host buffer that only grows (device-to-host staging)
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap) return GDSJ_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error("hipHostMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+            return GDSJ_EOOM;
+        }
+        cap = bytes;
+        return GDSJ_OK;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
 }  // namespace gdsj

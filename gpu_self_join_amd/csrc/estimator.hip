@@ -525,16 +525,16 @@ int estimator_capacity_sampling(const Grid &G, const void *joinArgs, int mode, u
 
 // ----------------------------------------------------------------- error --
 // one thread per tile: the tile's hits (a loop over its queries, 3 on
-// average at 6 dims) against the model's prediction, in float; a block's
-// sum and maximum go to the totals with one atomic each
+// average at 6 dims) against the model's prediction, in float; each block
+// writes its sum and maximum to part[block] (no contended atomics), and one
+// block reduces them
 constexpr uint32_t EST_BLOCK = 256;
 
 __global__ __launch_bounds__(EST_BLOCK) void k_est_error(const float *__restrict__ predLog2,
                                                          const uint4 *__restrict__ tiles, uint32_t nt,
                                                          const uint32_t *__restrict__ ids,
                                                          const uint32_t *__restrict__ counts,
-                                                         double *__restrict__ sumRel,
-                                                         unsigned int *__restrict__ maxRelBits) {
+                                                         float2 *__restrict__ part) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     float rel = 0.0f;
     if (t < nt) {
@@ -557,14 +557,44 @@ __global__ __launch_bounds__(EST_BLOCK) void k_est_error(const float *__restrict
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        float a = 0.0f, m = 0.0f;
+        for (uint32_t i = 0; i < EST_BLOCK / GDSJ_WAVE; i++) {
+            a += ps[i];
+            m = fmaxf(m, pm[i]);
+        }
+        part[blockIdx.x] = make_float2(a, m);
+    }
+}
+
+__global__ __launch_bounds__(EST_BLOCK) void k_est_error_reduce(const float2 *__restrict__ part, uint32_t np,
+                                                                double *__restrict__ out) {
+    double sm = 0.0;
+    float mx = 0.0f;
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
+        sm += (double)part[i].x;
+        mx = fmaxf(mx, part[i].y);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sm += __shfl_xor(sm, off, GDSJ_WAVE);
+        mx = fmaxf(mx, __shfl_xor(mx, off, GDSJ_WAVE));
+    }
+    __shared__ double ps[EST_BLOCK / GDSJ_WAVE];
+    __shared__ float pm[EST_BLOCK / GDSJ_WAVE];
+    const uint32_t w = threadIdx.x / GDSJ_WAVE;
+    if (lane_id() == 0) {
+        ps[w] = sm;
+        pm[w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
         double a = 0.0;
         float m = 0.0f;
         for (uint32_t i = 0; i < EST_BLOCK / GDSJ_WAVE; i++) {
             a += ps[i];
             m = fmaxf(m, pm[i]);
         }
-        atomicAdd(sumRel, a);
-        atomicMax(maxRelBits, __float_as_uint(m));  // non-negative floats order as their bits
+        out[0] = a;
+        out[1] = (double)m;
     }
 }
 
@@ -573,21 +603,18 @@ int estimator_error(const Grid &G, const float *predLog2, const uint32_t *counts
     const uint32_t nt = G.ntiles;
     *meanRel = *maxRel = 0.0;
     if (nt == 0) return GDSJ_OK;
-    GDSJ_TRY(scratch.ensure(64));
-    GDSJ_HIP(hipMemsetAsync(scratch.p, 0, 16, s));
-    double *sumRel = scratch.as<double>();
-    unsigned int *mx = (unsigned int *)(sumRel + 1);
-    k_est_error<<<ceil_div(nt, EST_BLOCK), EST_BLOCK, 0, s>>>(predLog2, G.tiles.as<uint4>(), nt,
-                                                              G.ids.as<uint32_t>(), counts, sumRel, mx);
+    const uint32_t nb = ceil_div(nt, EST_BLOCK);
+    GDSJ_TRY(scratch.ensure(64 + (size_t)nb * sizeof(float2)));
+    double *out = scratch.as<double>();
+    float2 *part = (float2 *)(scratch.as<char>() + 64);
+    k_est_error<<<nb, EST_BLOCK, 0, s>>>(predLog2, G.tiles.as<uint4>(), nt, G.ids.as<uint32_t>(), counts, part);
+    k_est_error_reduce<<<1, EST_BLOCK, 0, s>>>(part, nb, out);
+    GDSJ_HIP(hipGetLastError());
     double h[2] = {0, 0};
-    GDSJ_HIP(hipMemcpyAsync(h, scratch.p, 16, hipMemcpyDeviceToHost, s));
+    GDSJ_HIP(hipMemcpyAsync(h, out, 16, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipStreamSynchronize(s));
     *meanRel = h[0] / nt;
-    uint32_t bits;
-    memcpy(&bits, &h[1], 4);
-    float m;
-    memcpy(&m, &bits, 4);
-    *maxRel = m;
+    *maxRel = h[1];
     return GDSJ_OK;
 }
 

@@ -60,6 +60,7 @@ struct Grid {
     uint64_t runStride = 0;
     bool haveRuns = false;
     DevBuf keyA, keyB, valB, tmp, aux, flag, tileKey, tileVal, cellAux, scalars;
+    HostBuf hostSample;  // REORDER's sample rows, pinned (the variance is summed on the host)
     void release();
 };
 

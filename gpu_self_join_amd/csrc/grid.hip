@@ -32,98 +32,42 @@ struct KeyArgs {
 };
 
 // ------------------------------------------------------------- REORDER --
+// rows 0, 100, 200, ... of the points, row-major: the reference's variance
+// sample (main.cu:997-1009)
 template <typename T>
 __global__ void k_sample_gather(const T *__restrict__ data, uint32_t n, uint32_t d, uint32_t m,
                                 T *__restrict__ out) {
     uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (uint64_t)m * d) return;
     uint32_t j = (uint32_t)(idx / d), l = (uint32_t)(idx % d);
-    out[(uint64_t)l * m + j] = data[(uint64_t)j * 100u * d + l];
+    out[idx] = data[(uint64_t)j * 100u * d + l];
 }
 
-// One block of 256 threads per dimension: the reference's sequential sums,
-// unfused, in sample order, by lane 0 of wave 0.  The sample column streams
-// through LDS in chunks: waves 1..3 load chunk c + 1 (all of a thread's loads
-// issued before its stores) while lane 0 adds chunk c, so the dependent add
-// chain reads LDS instead of waiting on global memory.
-constexpr uint32_t VAR_LOADERS = 192;  // threads of waves 1..3
-constexpr uint32_t VAR_PER = 16;       // values per loader thread and chunk
-constexpr uint32_t VAR_CHUNK = VAR_LOADERS * VAR_PER;
-
+// The reference's per-dim mean and variance of the sample (main.cu:997-1022):
+// sequential sums in DTYPE, unfused, in sample order, each divided by
+// denom.  One chain per dim; the dims' chains are independent, so the
+// inner loop over dims may run in vector lanes without changing any sum.
+// Host code of a translation unit built with -ffp-contract=off.
 template <typename T>
-__device__ __forceinline__ void var_stage(const T *__restrict__ col, uint32_t m, uint32_t base, uint32_t t,
-                                          uint32_t nthr, T *dst) {
-    T v[VAR_PER];
-#pragma unroll
-    for (uint32_t u = 0; u < VAR_PER; u++) {
-        const uint32_t i = t + u * nthr;
-        v[u] = (i < VAR_CHUNK && base + i < m) ? col[base + i] : (T)0;
+static void sample_variance(const T *x, uint32_t m, uint32_t d, T denom, T *var) {
+    std::vector<T> sum(d, (T)0), avg(d);
+    for (uint32_t j = 0; j < m; j++) {
+        const T *row = x + (size_t)j * d;
+        for (uint32_t l = 0; l < d; l++) sum[l] = sum[l] + row[l];
     }
-#pragma unroll
-    for (uint32_t u = 0; u < VAR_PER; u++) {
-        const uint32_t i = t + u * nthr;
-        if (i < VAR_CHUNK && base + i < m) dst[i] = v[u];
+    for (uint32_t l = 0; l < d; l++) {
+        avg[l] = sum[l] / denom;
+        sum[l] = (T)0;
     }
-}
-
-template <typename T, bool SQ>
-__device__ __forceinline__ T var_add(T sum, T v, T avg) {
-    if (SQ) {
-        const T t = v - avg;
-        const T q = t * t;
-        return sum + q;
-    }
-    return sum + v;
-}
-
-template <typename T, bool SQ>
-__device__ __forceinline__ T var_chunk_sum(const T *buf, uint32_t cnt, T sum, T avg) {
-    // 32 values per step, read (16-byte LDS loads) before the step's adds
-    constexpr uint32_t U = 32;
-    uint32_t j = 0;
-    for (; j + U <= cnt; j += U) {
-        T v[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; u++) v[u] = buf[j + u];
-#pragma unroll
-        for (uint32_t u = 0; u < U; u++) sum = var_add<T, SQ>(sum, v[u], avg);
-    }
-    for (; j < cnt; j++) sum = var_add<T, SQ>(sum, buf[j], avg);
-    return sum;
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_variance(const T *__restrict__ samples, uint32_t m, T denom,
-                                                  T *__restrict__ var) {
-    __shared__ __align__(16) T buf[2][VAR_CHUNK];
-    __shared__ T s_avg;
-    const uint32_t l = blockIdx.x;
-    const T *col = samples + (uint64_t)l * m;
-    const uint32_t nchunk = (m + VAR_CHUNK - 1) / VAR_CHUNK;
-    T avg = 0;
-    for (int pass = 0; pass < 2; pass++) {
-        T sum = 0;
-        if (threadIdx.x >= GDSJ_WAVE) var_stage<T>(col, m, 0u, threadIdx.x - GDSJ_WAVE, VAR_LOADERS, buf[0]);
-        __syncthreads();
-        for (uint32_t c = 0; c < nchunk; c++) {
-            const uint32_t base = c * VAR_CHUNK;
-            if (threadIdx.x >= GDSJ_WAVE) {
-                if (c + 1 < nchunk)
-                    var_stage<T>(col, m, base + VAR_CHUNK, threadIdx.x - GDSJ_WAVE, VAR_LOADERS, buf[(c + 1) & 1]);
-            } else if (threadIdx.x == 0) {
-                const uint32_t cnt = m - base < VAR_CHUNK ? m - base : VAR_CHUNK;
-                sum = pass == 0 ? var_chunk_sum<T, false>(buf[c & 1], cnt, sum, avg)
-                                : var_chunk_sum<T, true>(buf[c & 1], cnt, sum, avg);
-            }
-            __syncthreads();
+    for (uint32_t j = 0; j < m; j++) {
+        const T *row = x + (size_t)j * d;
+        for (uint32_t l = 0; l < d; l++) {
+            const T t = row[l] - avg[l];
+            const T q = t * t;
+            sum[l] = sum[l] + q;
         }
-        if (threadIdx.x == 0) {
-            if (pass == 0) s_avg = sum / denom;
-            else var[l] = sum / denom;
-        }
-        __syncthreads();
-        avg = s_avg;
     }
+    for (uint32_t l = 0; l < d; l++) var[l] = sum[l] / denom;
 }
 
 // ------------------------------------------------------------ extent --
@@ -366,6 +310,7 @@ void Grid::release() {
                    &valB, &tmp, &aux, &flag, &tileKey, &tileVal, &cellAux, &scalars, &runs, &nruns, &cellCand, &cellHist,
                    &comboTab};
     for (DevBuf *p : b) p->release();
+    hostSample.release();
 }
 
 static int bit_length(uint64_t v) {
@@ -410,17 +355,18 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     for (uint32_t l = 0; l < d; l++) G.perm[l] = l;
     if (reorder && n > 0) {
         uint32_t m = (n + 99) / 100;
-        GDSJ_TRY(G.aux.ensure((size_t)m * d * sizeof(T) + d * sizeof(T) + 256));
+        const size_t sb = (size_t)m * d * sizeof(T);
+        GDSJ_TRY(G.aux.ensure(sb + 256));
+        GDSJ_TRY(G.hostSample.ensure(sb + 256));
         T *samp = G.aux.as<T>();
-        T *var = samp + (size_t)m * d;
         uint64_t tot = (uint64_t)m * d;
         k_sample_gather<T><<<(uint32_t)((tot + 255) / 256), 256, 0, s>>>(data, n, d, m, samp);
+        GDSJ_HIP(hipMemcpyAsync(G.hostSample.p, samp, sb, hipMemcpyDeviceToHost, s));
+        GDSJ_HIP(hipStreamSynchronize(s));
         const T inv_sample = (T)(1.0 / (100 * 1.0));
         const T denom = (T)n * inv_sample;
-        k_variance<T><<<d, 256, 0, s>>>(samp, m, denom, var);
         std::vector<T> hv(d);
-        GDSJ_HIP(hipMemcpyAsync(hv.data(), var, d * sizeof(T), hipMemcpyDeviceToHost, s));
-        GDSJ_HIP(hipStreamSynchronize(s));
+        sample_variance<T>((const T *)G.hostSample.p, m, d, denom, hv.data());
         // descending variance, ties in ascending dim order
         for (uint32_t i = 1; i < d; i++) {
             uint32_t pi = G.perm[i];

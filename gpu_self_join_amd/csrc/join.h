@@ -42,6 +42,13 @@ struct JoinArgs {
     GridArgs ga;
     uint32_t ncombo;   // 3^(k-1) neighbor combinations of dims 1..k-1
     double thr;        // largest DTYPE value x with sqrt(x) <= eps (exact in double)
+    // Gram-form filter of the plain kernels (every dim indexed): a pair is a
+    // neighbor when |c'|^2 - 2 q'.c' <= gvLo - |q'|^2 and not one when it
+    // exceeds gvHi - |q'|^2 (c', q': coordinates relative to the tile's
+    // first query); in between the reference's arithmetic decides
+    // (gram_valu_band).  gv = 0: the reference's arithmetic for every pair.
+    uint32_t gv;
+    float gvLo, gvMid, gvHalf;  // band [gvLo, gvLo + 2 half'], gvHalf >= half' with room
     uint32_t d;        // dims
     uint32_t ilp;      // ILP
     uint32_t sc;       // SHORTCIRCUIT early exit
@@ -98,6 +105,12 @@ constexpr uint32_t SPLIT_ROWS = 128;  // row stride of splitCnt / splitOff
 __host__ __device__ __forceinline__ uint32_t hmask_stride(uint32_t cand, uint32_t nruns) {
     return (cand + 63u) / 64u + (nruns + 63u) / 64u;
 }
+
+// Band around thr for the Gram-form filter of a join whose dims are all
+// indexed (FP32): pairs with g <= lo are neighbors, pairs with g > hi are
+// not, and |g - mid| <= half holds for every g in (lo, hi] even after the
+// rounding of g - mid.  False when the filter does not apply.
+bool gram_valu_band(const Grid &G, double thr, float *lo, float *mid, float *half);
 
 // Launches one pass (JoinPass) of the join over a.ntiles tiles on stream s.
 int join_pass(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t s);

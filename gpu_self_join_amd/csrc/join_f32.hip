@@ -6,4 +6,5 @@
     GDSJ_DIMCASE(8) GDSJ_DIMCASE(16) GDSJ_DIMCASE(32) GDSJ_DIMCASE(64)
 #define JOIN_DIMQ GDSJ_DIMQ(2) GDSJ_DIMQ(3) GDSJ_DIMQ(4) GDSJ_DIMQ(5) GDSJ_DIMQ(6) GDSJ_DIMQ(8) \
     GDSJ_DIMQ(16) GDSJ_DIMQ(32) GDSJ_DIMQ(64)
+#define JOIN_GV_DIM(D) ((D) >= 3 && (D) <= 8)
 #include "join_inst.h"

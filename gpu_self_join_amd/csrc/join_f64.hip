@@ -5,4 +5,5 @@
 #define JOIN_DIMS GDSJ_DIMCASE(2) GDSJ_DIMCASE(4) GDSJ_DIMCASE(8) GDSJ_DIMCASE(16) GDSJ_DIMCASE(32) \
     GDSJ_DIMCASE(64)
 #define JOIN_DIMQ GDSJ_DIMQ(2) GDSJ_DIMQ(4) GDSJ_DIMQ(8) GDSJ_DIMQ(16) GDSJ_DIMQ(32) GDSJ_DIMQ(64)
+#define JOIN_GV_DIM(D) false
 #include "join_inst.h"

@@ -4,7 +4,7 @@
 
 namespace gdsj {
 
-template <int D, int MODE, int ILPV>
+template <int D, int MODE, int ILPV, bool GV>
 static int launch_join_t(const Grid &G, const JoinArgs &a, int pass, hipStream_t s) {
     const uint32_t waves = 256 / GDSJ_WAVE;
     const uint32_t blocks = (a.ntiles + waves - 1) / waves;
@@ -18,24 +18,33 @@ static int launch_join_t(const Grid &G, const JoinArgs &a, int pass, hipStream_t
     const bool hm = a.hmask != nullptr && !a.stamp && a.runs != nullptr;
     const uint64_t *hmp = a.hmask;
     if (pass == PASS_FILL) {
-        if (a.stamp) k_join<JT, D, MODE, ILPV, PASS_FILL, true><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
+        if (a.stamp) k_join<JT, D, MODE, ILPV, PASS_FILL, true, false, GV><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
         else if (hm) k_join<JT, D, MODE, ILPV, PASS_FILL, false, true><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
-        else k_join<JT, D, MODE, ILPV, PASS_FILL><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
+        else k_join<JT, D, MODE, ILPV, PASS_FILL, false, false, GV><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
     } else if (pass == PASS_SLOT) {
-        k_join<JT, D, MODE, ILPV, PASS_SLOT><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
+        k_join<JT, D, MODE, ILPV, PASS_SLOT, false, false, GV><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
     } else {
-        if (a.stamp) k_join<JT, D, MODE, ILPV, PASS_COUNT, true><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
-        else if (hm) k_join<JT, D, MODE, ILPV, PASS_COUNT, false, true><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
-        else k_join<JT, D, MODE, ILPV, PASS_COUNT><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
+        if (a.stamp) k_join<JT, D, MODE, ILPV, PASS_COUNT, true, false, GV><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
+        else if (hm) k_join<JT, D, MODE, ILPV, PASS_COUNT, false, true, GV><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
+        else k_join<JT, D, MODE, ILPV, PASS_COUNT, false, false, GV><<<blocks, 256, 0, s>>>(xs, a.ids, a, hmp);
     }
     GDSJ_HIP(hipGetLastError());
     return GDSJ_OK;
 }
 
+// the Gram-form filter variants exist for FP32 and the dims JOIN_GV_DIM accepts
+template <int D, int MODE, int ILPV>
+static int launch_join_gv(const Grid &G, const JoinArgs &a, int pass, hipStream_t s) {
+    if constexpr (JOIN_GV_DIM(D)) {
+        if (a.gv) return launch_join_t<D, MODE, ILPV, true>(G, a, pass, s);
+    }
+    return launch_join_t<D, MODE, ILPV, false>(G, a, pass, s);
+}
+
 #define GDSJ_DIMCASE(DV)                                                              \
     case DV:                                                                          \
-        if (mode == ACC_SEQ) return launch_join_t<DV, ACC_SEQ, 1>(G, a, pass, s);     \
-        if (mode == ACC_ILP_SC && a.ilp == 8) return launch_join_t<DV, ACC_ILP_SC, 8>(G, a, pass, s); \
+        if (mode == ACC_SEQ) return launch_join_gv<DV, ACC_SEQ, 1>(G, a, pass, s);    \
+        if (mode == ACC_ILP_SC && a.ilp == 8) return launch_join_gv<DV, ACC_ILP_SC, 8>(G, a, pass, s); \
         break;
 
 int JNAME(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t s) {
@@ -44,9 +53,9 @@ int JNAME(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t s) {
         default:
             break;
     }
-    if (mode == ACC_SEQ) return launch_join_t<0, ACC_SEQ, 0>(G, a, pass, s);
-    if (mode == ACC_ILP_SC) return launch_join_t<0, ACC_ILP_SC, 0>(G, a, pass, s);
-    return launch_join_t<0, ACC_ILP_NOSC, 0>(G, a, pass, s);
+    if (mode == ACC_SEQ) return launch_join_t<0, ACC_SEQ, 0, false>(G, a, pass, s);
+    if (mode == ACC_ILP_SC) return launch_join_t<0, ACC_ILP_SC, 0, false>(G, a, pass, s);
+    return launch_join_t<0, ACC_ILP_NOSC, 0, false>(G, a, pass, s);
 }
 
 bool JNAME_SPECIALISED(uint32_t d, int mode, uint32_t ilp) {

@@ -256,7 +256,7 @@ __device__ __forceinline__ T dist_row(const T *__restrict__ qrow, const T *__res
 // hits from them and loads no coordinates.  Both passes cut the candidates
 // into the same windows and chunk batches, so chunk numbers agree.
 template <typename T, int D, int MODE, int ILPV, int PASS, int CM = 0, bool STAMP = false, int QRW = 16,
-          bool UNSW = true, bool BMAP = true, bool HM = false>
+          bool UNSW = true, bool BMAP = true, bool HM = false, bool GV = false>
 __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
                                           const JoinArgs &a, const uint32_t tile,
                                           const uint64_t *__restrict__ hmr = nullptr) {
@@ -307,10 +307,28 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
     // low dims: lane l holds query l's coordinates; the query loop reads them
     // with readlane (no memory access per query)
     constexpr bool QREG = !MFILL && D > 0 && D * (int)(sizeof(T) / 4) <= QRW;
+    // Gram-form filter (GV): coordinates relative to the tile's first query
+    // (go, wave-uniform); lane l holds -2 (q_l - go) of query l (qv), and
+    // gvLo and the band's midpoint gvMid less |q_l - go|^2 (tqa, tqm)
+    constexpr bool GVF = GV && QREG && !MFILL;
     T qv[QREG ? D : 1];
+    T go[GVF ? D : 1];
+    T tqa = 0, tqm = 0;
     if (QREG) {
 #pragma unroll
         for (int l = 0; l < (QREG ? D : 1); l++) qv[l] = lane < nq ? xs[(uint64_t)l * ld + q0 + lane] : (T)0;
+    }
+    if (GVF) {
+        T qn = 0;
+#pragma unroll
+        for (int l = 0; l < (GVF ? D : 1); l++) {
+            go[l] = lane_value(qv[l], 0u);
+            const T b = qv[l] - go[l];
+            qn = l == 0 ? b * b : fma_rn(b, b, qn);
+            qv[l] = (T)-2 * b;
+        }
+        tqa = (T)a.gvLo - qn;
+        tqm = (T)a.gvMid - qn;
     }
     uint32_t cnt_l = 0;
     uint64_t cur_l = 0;  // fill pass: this lane's query's next CSR slot
@@ -387,6 +405,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
             T c[CMAX][D > 0 ? D : 1];
             uint32_t cpos[CMAX];
             uint32_t cid[CMAX];
+            T cn[GVF ? CMAX : 1];  // GV: |c'|^2 of this lane's candidate
             bool valid[CMAX];
             bool other[CMAX];                                    // STAMP: candidate outside the own cell
             uint32_t jc[STAMP && PASS == PASS_COUNT ? CMAX : 1];  // STAMP: hits of this candidate
@@ -419,6 +438,15 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                     const uint64_t xp = valid[ch] ? (uint64_t)sp : ld - 1;
 #pragma unroll
                     for (int l = 0; l < (D > 0 ? D : 1); l++) c[ch][l] = xs[(uint64_t)l * ld + xp];
+                    if (GVF) {
+                        T nn = 0;
+#pragma unroll
+                        for (int l = 0; l < (GVF ? D : 1); l++) {
+                            c[ch][l] = c[ch][l] - go[l];
+                            nn = l == 0 ? c[ch][l] * c[ch][l] : fma_rn(c[ch][l], c[ch][l], nn);
+                        }
+                        cn[ch] = nn;
+                    }
                 }
                 if (NEED_IDS) cid[ch] = (ch < (int)nch) ? ids[sp] : 0u;
             }
@@ -426,11 +454,21 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
             // (the single-join kernel); one copy with the check otherwise
             if (UNSW && PASS == PASS_FILL && roomy) {
 #define GDSJ_RY true
+#define GDSJ_FULL false
 #include "join_query_loop.inc"
+#undef GDSJ_FULL
+#undef GDSJ_RY
+            } else if (GVF && nch == (uint32_t)CMAX) {
+#define GDSJ_RY (UNSW ? false : roomy)
+#define GDSJ_FULL true
+#include "join_query_loop.inc"
+#undef GDSJ_FULL
 #undef GDSJ_RY
             } else {
 #define GDSJ_RY (UNSW ? false : roomy)
+#define GDSJ_FULL false
 #include "join_query_loop.inc"
+#undef GDSJ_FULL
 #undef GDSJ_RY
             }
             if (PASS == PASS_FILL) cur_l += inc_l;
@@ -478,15 +516,20 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
 }
 
 // hm: the hit masks (HM; a.hmask), read-only here in the fill pass
-template <typename T, int D, int MODE, int ILPV, int PASS, bool STAMP = false, bool HM = false>
+// GV: the Gram-form filter (JoinArgs::gv) decides the pairs outside its band
+template <typename T, int D, int MODE, int ILPV, int PASS, bool STAMP = false, bool HM = false, bool GV = false>
 __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
                                               const JoinArgs a, const uint64_t *__restrict__ hm) {
     const uint32_t lblock = a.xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t tslot = uniform((uint32_t)(lblock * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE));
     if (tslot >= a.ntiles) return;
     const uint32_t tile = a.tileList ? uniform(a.tileList[tslot]) : tslot;
-    constexpr int CM = PASS == PASS_SLOT ? CMaxFor<T, D, GDSJ_CMAX_WORDS_SLOT>::value : 0;
-    join_tile<T, D, MODE, ILPV, PASS, CM, STAMP, 16, true, true, HM>(xs, ids, a, tile, hm);
+    // the Gram-form filter holds one more word per candidate (|c'|^2); the
+    // hit-mask passes keep the default so that the counting pass and the
+    // fill from its masks cut the candidates into the same chunks
+    constexpr int CM = PASS == PASS_SLOT ? CMaxFor<T, GV ? D + 1 : D, GDSJ_CMAX_WORDS_SLOT>::value
+                                         : (GV && !HM ? CMaxFor<T, D + 1>::value : 0);
+    join_tile<T, D, MODE, ILPV, PASS, CM, STAMP, 16, true, true, HM, GV>(xs, ids, a, tile, hm);
 }
 
 }  // namespace gdsj

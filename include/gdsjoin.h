@@ -90,6 +90,11 @@ typedef struct gdsj_params {
                                into the running sum, which is the arithmetic of ILP = 0.  The
                                two readings differ only on pairs whose sums straddle the
                                threshold by a few ulps */
+    uint32_t vfilter;       /* Gram-form prefilter of the plain FP32 kernels when every dim is
+                               indexed (3 to 8 dims): pairs are decided from |c'|^2 - 2 q'.c'
+                               in tile-local coordinates, and those within a proven error
+                               band of the threshold by the reference's arithmetic: 0 auto
+                               (off), 1 on where it applies, 2 off (speed only) */
 } gdsj_params;
 
 /* Timing and diagnostics of the last join on a handle. */

@@ -365,3 +365,27 @@ def test_lattice_on_cell_faces(sj, d, k, dt, jitter):
     x = x.astype(dt)
     for ilp, sc, reorder in [(0, 0, 0), (8, 1, 1)]:
         _check(sj, x, float(np.asarray(eps, dtype=dt)), k=k, ilp=ilp, sc=sc, reorder=reorder)
+
+
+@pytest.mark.parametrize("d", [3, 4, 6, 8])
+def test_vfilter_on_off_same_table(sj, d):
+    """The Gram-form prefilter of the plain FP32 kernels (every dim indexed)
+    gives the oracle's table, on clustered data and on an integer lattice
+    whose pairs at distance exactly eps all fall inside the filter's band
+    (the reference's arithmetic decides them)."""
+    rng = np.random.default_rng(90 + d)
+    centers = rng.random((6, d)) * 12
+    x = (centers[rng.integers(0, 6, 5000)] + rng.standard_normal((5000, d))).astype(np.float32)
+    eps = _eps_for(x, 40)
+    side = {3: 14, 4: 8, 6: 4, 8: 3}[d]
+    grid = np.stack(np.meshgrid(*[np.arange(side, dtype=np.float32)] * d, indexing="ij"), -1).reshape(-1, d)
+    for data, e in ((x, eps), (grid + np.float32(20.0), 1.0)):
+        for vf in (1, 2):
+            for kw in (dict(ilp=8, shortcircuit=1), dict(ilp=0, shortcircuit=0)):
+                c, ids, r = _gpu_join(sj, data, e, k=d, reorder=0, vfilter=vf, **kw)
+                ref = O.join(data, e, k=d, reorder=0, **kw)
+                assert_csr_equal(c, ids, ref["counts"], ref["ids"])
+            for est in (_lib.EST_LEARNED, _lib.EST_LEARNED):  # train, then the slotted pass
+                c, ids, r = _gpu_join(sj, data, e, k=d, reorder=0, vfilter=vf, ilp=8, shortcircuit=1, estimator=est)
+            ref = O.join(data, e, k=d, reorder=0, ilp=8, shortcircuit=1)
+            assert_csr_equal(c, ids, ref["counts"], ref["ids"])
