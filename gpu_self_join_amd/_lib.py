@@ -34,7 +34,10 @@ EXPORTS = [
     "gdsj_model_get", "gdsj_model_set",
     "gdsj_batch_create", "gdsj_batch_destroy", "gdsj_batch_autotune", "gdsj_batch_run", "gdsj_batch_get_stats",
     "gdsj_csv_load", "gdsj_bruteforce_device", "gdsj_bruteforce_host", "gdsj_rejoin_device",
+    "gdsj_bruteforce_device_noalloc",
 ]
+
+NOALLOC_SCRATCH_BYTES = 32 << 20  # GDSJ_NOALLOC_SCRATCH_BYTES
 
 ILP_AUTO = 0xFFFFFFFF
 
@@ -153,6 +156,8 @@ def load():
     L.gdsj_bruteforce_device.argtypes = [P, P, u32, u32, i32, C.c_double, C.POINTER(GdsjParams), P, P, P, u64,
                                          C.POINTER(u64), P]
     L.gdsj_rejoin_device.argtypes = [P, P, P, P, u64, C.POINTER(u64), P]
+    L.gdsj_bruteforce_device_noalloc.argtypes = [P, u32, u32, i32, C.c_double, C.POINTER(GdsjParams), P, P, P,
+                                                 u64, C.POINTER(u64), P]
     L.gdsj_bruteforce_host.argtypes = [P, P, u32, u32, i32, C.c_double, C.POINTER(GdsjParams), P,
                                        C.POINTER(u64)]
     _lib = L

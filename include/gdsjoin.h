@@ -193,6 +193,25 @@ int gdsj_bruteforce_host(gdsj_handle *h, const void *data, uint32_t n, uint32_t 
                          int dtype, double epsilon, const gdsj_params *p,
                          uint32_t *counts, uint64_t *total);
 
+/* Allocation-free all-pairs join.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It runs
+ * gdsj_bruteforce_device's kernels with every scratch buffer (the SoA copy of
+ * the points, the scan's partial sums) in one fixed device array of
+ * GDSJ_NOALLOC_SCRATCH_BYTES that each device holds from the library's load
+ * to the end of the process; it calls no allocating routine.  The caller
+ * passes the points, counts[n], offsets[n+1] (required) and, to get the rows,
+ * ids[ids_cap].  Inputs needing more scratch than the array holds
+ * (about n * dims * sizeof(DTYPE) bytes) are rejected with GDSJ_EINVAL.
+ * Runs on the current device and the given stream (NULL: the null stream)
+ * and returns after the stream has drained.  Calls that overlap on one
+ * device race on the shared array. */
+#define GDSJ_NOALLOC_SCRATCH_BYTES (32u << 20)
+int gdsj_bruteforce_device_noalloc(const void *d_data, uint32_t n, uint32_t dims,
+                                   int dtype, double epsilon, const gdsj_params *p,
+                                   uint32_t *d_counts, uint64_t *d_offsets, uint32_t *d_ids,
+                                   uint64_t ids_cap, uint64_t *total, void *stream);
+
 /* Largest x with sqrt_rn(x) <= eps, the threshold that replaces the
  * reference's per-pair square root (kernel.cu:361,364). */
 float gdsj_eps_threshold_f32(float eps);
