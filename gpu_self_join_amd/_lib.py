@@ -34,7 +34,7 @@ EXPORTS = [
     "gdsj_model_get", "gdsj_model_set",
     "gdsj_batch_create", "gdsj_batch_destroy", "gdsj_batch_autotune", "gdsj_batch_run", "gdsj_batch_get_stats",
     "gdsj_csv_load", "gdsj_bruteforce_device", "gdsj_bruteforce_host", "gdsj_rejoin_device",
-    "gdsj_bruteforce_device_noalloc",
+    "gdsj_bruteforce_device_noalloc", "gdsj_bruteforce_rows_host",
 ]
 
 NOALLOC_SCRATCH_BYTES = 32 << 20  # GDSJ_NOALLOC_SCRATCH_BYTES
@@ -49,7 +49,8 @@ class GdsjParams(C.Structure):
                 ("verbose", C.c_uint32), ("tile_queries", C.c_uint32), ("schedule", C.c_uint32),
                 ("prune", C.c_uint32), ("train", C.c_uint32), ("filter_stats", C.c_uint32),
                 ("hitmask", C.c_uint32), ("ilp_fold", C.c_uint32),
-                ("vfilter", C.c_uint32)]
+                ("vfilter", C.c_uint32), ("sort", C.c_uint32),
+                ("cpu_sum", C.c_uint32)]
 
 
 class GdsjStats(C.Structure):
@@ -156,6 +157,7 @@ def load():
     L.gdsj_bruteforce_device.argtypes = [P, P, u32, u32, i32, C.c_double, C.POINTER(GdsjParams), P, P, P, u64,
                                          C.POINTER(u64), P]
     L.gdsj_rejoin_device.argtypes = [P, P, P, P, u64, C.POINTER(u64), P]
+    L.gdsj_bruteforce_rows_host.argtypes = [P, P, u64]
     L.gdsj_bruteforce_device_noalloc.argtypes = [P, u32, u32, i32, C.c_double, C.POINTER(GdsjParams), P, P, P,
                                                  u64, C.POINTER(u64), P]
     L.gdsj_bruteforce_host.argtypes = [P, P, u32, u32, i32, C.c_double, C.POINTER(GdsjParams), P,

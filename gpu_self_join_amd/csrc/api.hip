@@ -13,8 +13,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <cmath>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "bruteforce.h"
@@ -40,6 +42,14 @@ void set_error(const char *fmt, ...) {
 
 using namespace gdsj;
 
+// Host-buffer copies (gdsjoin's path) run through pinned staging:
+// STAGE_THREADS host threads each move every STAGE_THREADS-th chunk of
+// STAGE_CHUNK bytes through two pinned buffers on a stream of their own, so
+// that DMA of one chunk overlaps the host memcpy of the previous one and the
+// threads' transfers overlap each other.
+constexpr int STAGE_THREADS = 8;
+constexpr size_t STAGE_CHUNK = 8u << 20;
+
 struct gdsj_handle {
     int device = 0;
     hipStream_t own = nullptr;
@@ -62,6 +72,12 @@ struct gdsj_handle {
     bool slotted = false;
     bool have_grid = false;  // a grid, runs and tiles of the last join (gdsj_rejoin_device)
     uint32_t novf = 0;
+    // the last all-pairs join's setting, for gdsj_bruteforce_rows_host
+    bool bf_ok = false;
+    uint32_t bf_n = 0, bf_dims = 0, bf_ilp = 0;
+    int bf_dtype = 0, bf_mode = 0;
+    double bf_thr = 0;
+    uint64_t bf_ld = 0, bf_total = 0;
     gdsj_params p{};
     int mode = 0;
     double thr = 0;
@@ -72,7 +88,14 @@ struct gdsj_handle {
     uint64_t *cur_offsets = nullptr;
     gdsj_stats st{};
     hipEvent_t ev[9] = {};  // 8: start of the row sort
+    // host-buffer copies: per copy thread a stream, two pinned chunk buffers
+    // and their events (staged_copy)
+    hipStream_t cstream[STAGE_THREADS] = {};
+    hipEvent_t cev[STAGE_THREADS][2] = {};
+    HostBuf pin[STAGE_THREADS][2];
 };
+
+static int staged_copy(gdsj_handle *h, void *dst, const void *src, size_t bytes, bool d2h);
 
 extern "C" {
 
@@ -170,6 +193,14 @@ int gdsj_destroy(gdsj_handle *h) {
     for (DevBuf *x : b) x->release();
     for (int i = 0; i < 9; i++)
         if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
+    for (int t = 0; t < STAGE_THREADS; t++) {
+        if (h->cstream[t]) (void)hipStreamSynchronize(h->cstream[t]);
+        for (int b = 0; b < 2; b++) {
+            if (h->cev[t][b]) (void)hipEventDestroy(h->cev[t][b]);
+            h->pin[t][b].release();
+        }
+        if (h->cstream[t]) (void)hipStreamDestroy(h->cstream[t]);
+    }
     if (h->own) (void)hipStreamDestroy(h->own);
     delete h;
     return GDSJ_OK;
@@ -185,6 +216,22 @@ static int resolve_params(const gdsj_params *in, uint32_t dims, gdsj_params *out
     if (out->ilp > 64) {
         set_error("ILP=%u exceeds 64", out->ilp);
         return GDSJ_EINVAL;
+    }
+    if (out->cpu_sum) {
+        set_error("cpu_sum applies to the all-pairs join only");
+        return GDSJ_EUNSUPPORTED;
+    }
+    if (out->sort > 1) {
+        set_error("sort=%u: 0 off, 1 on", out->sort);
+        return GDSJ_EINVAL;
+    }
+    if (out->sort && out->stamp) {
+        set_error("SORT runs without unicomp (stamp 0)");
+        return GDSJ_EUNSUPPORTED;
+    }
+    if (out->sort && (out->filter == GDSJ_FILTER_GRAM || out->filter == GDSJ_FILTER_GRAM_BF16)) {
+        set_error("SORT runs with the plain distance kernels (filter PLAIN or AUTO)");
+        return GDSJ_EUNSUPPORTED;
     }
     if (out->vfilter > 2) {
         set_error("vfilter=%u: 0 auto, 1 on, 2 off", out->vfilter);
@@ -239,7 +286,12 @@ static JoinArgs make_args(gdsj_handle *h) {
     // the Gram-form prefilter on request (auto leaves it off: on gmm6d's
     // slotted pass it issues 30 % fewer VALU instructions but runs 15 %
     // slower, its per-chunk band test and branches serialising the waves)
-    if (h->p.vfilter == 1 && gram_valu_band(h->G, h->thr, &a.gvLo, &a.gvMid, &a.gvHalf)) a.gv = 1;
+    if (h->p.vfilter == 1 && !h->p.sort && gram_valu_band(h->G, h->thr, &a.gvLo, &a.gvMid, &a.gvHalf)) a.gv = 1;
+    if (h->p.sort) {
+        a.sortMode = h->G.k == h->G.d ? 1u : 2u;
+        a.sortDim = h->G.k == h->G.d ? 0u : h->G.k;
+        a.posCell = h->G.posCell.as<uint32_t>();
+    }
     if (h->G.haveRuns) {
         a.runs = h->G.runs.as<uint2>();
         a.nruns = h->G.nruns.as<uint32_t>();
@@ -456,7 +508,7 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
         return GDSJ_EINVAL;
     }
     const bool gram_asked = h->p.filter == GDSJ_FILTER_GRAM || h->p.filter == GDSJ_FILTER_GRAM_BF16;
-    h->use_gram = (gram_asked || h->p.filter == GDSJ_FILTER_AUTO) && gram_supported(dims);
+    h->use_gram = (gram_asked || h->p.filter == GDSJ_FILTER_AUTO) && gram_supported(dims) && !h->p.sort;
     // operands: fp16 (one MFMA per k-step) unless bf16 hi + lo is asked for FP32 data
     const int gram_lowp = (h->p.filter == GDSJ_FILTER_GRAM_BF16 && dtype == GDSJ_DTYPE_F32) ? 0 : 1;
     if (gram_asked && !h->use_gram) {
@@ -478,6 +530,7 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     } else if (tq > 64) {
         tq = 64;  // the plain kernels keep one query per lane
     }
+    h->G.sortCells = h->p.sort ? 1 : 0;
     GDSJ_TRY(grid_build(h->G, d_data, n, dims, dtype, epsilon, h->p.k, h->p.reorder, tq, h->p.queryreorder, s));
     if (h->use_gram) {
         GDSJ_TRY(gram_prepare(h->gram, h->G, gram_lowp, s));
@@ -487,7 +540,8 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     // (auto: from 4 dims, where a pair's distance costs more than reading
     // its mask bit back; at 2 dims the fill pass is faster recomputing)
     h->use_hm = false;
-    const bool hm_ok = !h->use_gram && !h->stamp && h->p.estimator == GDSJ_EST_EXACT && !h->p.filter_stats;
+    const bool hm_ok = !h->use_gram && !h->stamp && h->p.estimator == GDSJ_EST_EXACT && !h->p.filter_stats &&
+                       !h->p.sort;
     if (h->p.hitmask == 1 && !hm_ok) {
         set_error("hit masks need the exact passes without the Gram filter and without unicomp");
         return GDSJ_EINVAL;
@@ -673,9 +727,10 @@ int gdsj_bruteforce_device(gdsj_handle *h, const void *d_data, uint32_t n, uint3
     } else {
         thr = gdsj_eps_threshold_f64(epsilon);
     }
-    const int mode = acc_mode(p.ilp, p.shortcircuit, p.ilp_fold);
+    const int mode = p.cpu_sum ? ACC_UNFUSED : acc_mode(p.ilp, p.shortcircuit, p.ilp_fold);
     GDSJ_HIP(hipSetDevice(h->device));
     h->have_counts = h->have_table = false;
+    h->bf_ok = false;
     hipStream_t s = stream ? (hipStream_t)stream : h->own;
     const size_t es = dtype == GDSJ_DTYPE_F64 ? 8 : 4;
     const uint64_t ld = ((uint64_t)n + 63) / 64 * 64 + 64;
@@ -694,6 +749,16 @@ int gdsj_bruteforce_device(gdsj_handle *h, const void *d_data, uint32_t n, uint3
     GDSJ_HIP(hipMemcpyAsync(&tot, offs + n, 8, hipMemcpyDeviceToHost, s));
     GDSJ_HIP(hipStreamSynchronize(s));
     *total = tot;
+    // what gdsj_bruteforce_rows_host needs to fill the rows later
+    h->bf_ok = !d_offsets;
+    h->bf_n = n;
+    h->bf_dims = dims;
+    h->bf_dtype = dtype;
+    h->bf_mode = mode;
+    h->bf_ilp = p.ilp;
+    h->bf_thr = thr;
+    h->bf_ld = ld;
+    h->bf_total = tot;
     if (!d_ids) return GDSJ_OK;
     if (tot > ids_cap) {
         set_error("result has %llu pairs, output buffer holds %llu", (unsigned long long)tot,
@@ -714,12 +779,32 @@ int gdsj_bruteforce_host(gdsj_handle *h, const void *data, uint32_t n, uint32_t 
     GDSJ_TRY(h->data.ensure(bytes + 256));
     GDSJ_TRY(h->counts.ensure((size_t)n * 4 + 256));
     hipStream_t s = h->own;
-    if (bytes) GDSJ_HIP(hipMemcpyAsync(h->data.p, data, bytes, hipMemcpyHostToDevice, s));
+    GDSJ_HIP(hipStreamSynchronize(s));
+    GDSJ_TRY(staged_copy(h, h->data.p, data, bytes, false));
     GDSJ_TRY(gdsj_bruteforce_device(h, h->data.p, n, dims, dtype, epsilon, p, h->counts.as<uint32_t>(), nullptr,
                                     nullptr, 0, total, s));
-    if (n) GDSJ_HIP(hipMemcpyAsync(counts, h->counts.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
-    GDSJ_HIP(hipStreamSynchronize(s));
+    GDSJ_TRY(staged_copy(h, counts, h->counts.p, (size_t)n * 4, true));
     return GDSJ_OK;
+}
+
+int gdsj_bruteforce_rows_host(gdsj_handle *h, uint32_t *ids, uint64_t cap) {
+    if (!h) return GDSJ_EINVAL;
+    if (!h->bf_ok) {
+        set_error("gdsj_bruteforce_rows_host needs a preceding gdsj_bruteforce_host on this handle");
+        return GDSJ_ESTATE;
+    }
+    if (cap < h->bf_total || (!ids && h->bf_total)) {
+        set_error("buffer holds %llu ids, the table has %llu", (unsigned long long)cap,
+                  (unsigned long long)h->bf_total);
+        return GDSJ_EOVERFLOW;
+    }
+    GDSJ_HIP(hipSetDevice(h->device));
+    hipStream_t s = h->own;
+    GDSJ_TRY(h->ids.ensure((size_t)h->bf_total * 4 + 256));
+    GDSJ_TRY(bruteforce_pass(h->bfXs.p, h->bf_ld, h->bf_n, h->bf_dims, h->bf_dtype, h->bf_mode, h->bf_ilp, h->bf_thr,
+                             PASS_FILL, nullptr, h->offsets.as<uint64_t>(), h->ids.as<uint32_t>(), s));
+    GDSJ_HIP(hipStreamSynchronize(s));
+    return staged_copy(h, ids, h->ids.p, (size_t)h->bf_total * 4, true);
 }
 
 int gdsj_fill_device(gdsj_handle *h, uint32_t *d_ids, uint64_t ids_cap) {
@@ -739,6 +824,88 @@ int gdsj_fill_device(gdsj_handle *h, uint32_t *d_ids, uint64_t ids_cap) {
     return GDSJ_OK;
 }
 
+// Copies bytes between a host buffer and device memory of h's device
+// (d2h: device src -> host dst) through the pinned staging buffers; returns
+// when the copy is complete.  Small copies go directly.
+static int staged_copy(gdsj_handle *h, void *dst, const void *src, size_t bytes, bool d2h) {
+    if (bytes == 0) return GDSJ_OK;
+    if (bytes < 2 * STAGE_CHUNK) {
+        GDSJ_HIP(hipMemcpy(dst, src, bytes, d2h ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice));
+        return GDSJ_OK;
+    }
+    const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
+    const int nt = (int)(nch < (size_t)STAGE_THREADS ? nch : (size_t)STAGE_THREADS);
+    for (int t = 0; t < nt; t++) {
+        if (!h->cstream[t]) GDSJ_HIP(hipStreamCreateWithFlags(&h->cstream[t], hipStreamNonBlocking));
+        for (int b = 0; b < 2; b++) {
+            if (!h->cev[t][b]) GDSJ_HIP(hipEventCreateWithFlags(&h->cev[t][b], hipEventDisableTiming));
+            GDSJ_TRY(h->pin[t][b].ensure(STAGE_CHUNK));
+        }
+    }
+    std::atomic<int> failed{0};
+    auto work = [&](int t) {
+        if (hipSetDevice(h->device) != hipSuccess) {
+            failed = 1;
+            return;
+        }
+        hipStream_t cs = h->cstream[t];
+        size_t prev_off = 0, prev_len = 0;
+        int i = 0;
+        for (size_t c = (size_t)t; c < nch; c += (size_t)nt, i++) {
+            const int b = i & 1;
+            const size_t off = c * STAGE_CHUNK;
+            const size_t len = bytes - off < STAGE_CHUNK ? bytes - off : STAGE_CHUNK;
+            void *pb = h->pin[t][b].p;
+            if (d2h) {
+                if (hipMemcpyAsync(pb, (const char *)src + off, len, hipMemcpyDeviceToHost, cs) != hipSuccess ||
+                    hipEventRecord(h->cev[t][b], cs) != hipSuccess) {
+                    failed = 1;
+                    return;
+                }
+                if (i > 0) {
+                    if (hipEventSynchronize(h->cev[t][b ^ 1]) != hipSuccess) {
+                        failed = 1;
+                        return;
+                    }
+                    memcpy((char *)dst + prev_off, h->pin[t][b ^ 1].p, prev_len);
+                }
+                prev_off = off;
+                prev_len = len;
+            } else {
+                // buffer b was last handed to the DMA two chunks ago
+                if (i >= 2 && hipEventSynchronize(h->cev[t][b]) != hipSuccess) {
+                    failed = 1;
+                    return;
+                }
+                memcpy(pb, (const char *)src + off, len);
+                if (hipMemcpyAsync((char *)dst + off, pb, len, hipMemcpyHostToDevice, cs) != hipSuccess ||
+                    hipEventRecord(h->cev[t][b], cs) != hipSuccess) {
+                    failed = 1;
+                    return;
+                }
+            }
+        }
+        if (d2h && i > 0) {
+            if (hipEventSynchronize(h->cev[t][(i - 1) & 1]) != hipSuccess) {
+                failed = 1;
+                return;
+            }
+            memcpy((char *)dst + prev_off, h->pin[t][(i - 1) & 1].p, prev_len);
+        }
+        if (hipStreamSynchronize(cs) != hipSuccess) failed = 1;
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+    GDSJ_HIP(hipSetDevice(h->device));
+    if (failed) {
+        set_error("staged %s copy of %zu bytes failed", d2h ? "device-to-host" : "host-to-device", bytes);
+        return GDSJ_EDEVICE;
+    }
+    return GDSJ_OK;
+}
+
 int gdsj_join_host(gdsj_handle *h, const void *data, uint32_t n, uint32_t dims, int dtype, double epsilon,
                    const gdsj_params *p, uint32_t *counts, uint64_t *total) {
     if (!h || (!data && n) || (!counts && n) || !total) return GDSJ_EINVAL;
@@ -748,14 +915,15 @@ int gdsj_join_host(gdsj_handle *h, const void *data, uint32_t n, uint32_t dims, 
     GDSJ_TRY(h->data.ensure(bytes + 256));
     GDSJ_TRY(h->counts.ensure((size_t)n * 4 + 256));
     hipStream_t s = h->own;
-    if (bytes) GDSJ_HIP(hipMemcpyAsync(h->data.p, data, bytes, hipMemcpyHostToDevice, s));
+    GDSJ_HIP(hipStreamSynchronize(s));  // the previous join no longer reads data
+    GDSJ_TRY(staged_copy(h, h->data.p, data, bytes, false));
     uint64_t tot = 0;
     int rc = gdsj_join_device(h, h->data.p, n, dims, dtype, epsilon, p, h->counts.as<uint32_t>(), nullptr,
                               nullptr, 0, &tot, s);
     if (rc != GDSJ_OK && rc != GDSJ_EOVERFLOW) return rc;
     GDSJ_TRY(h->ids.ensure((size_t)tot * 4 + 256));
     GDSJ_TRY(gdsj_fill_device(h, h->ids.as<uint32_t>(), tot));
-    if (n) GDSJ_HIP(hipMemcpy(counts, h->counts.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    GDSJ_TRY(staged_copy(h, counts, h->counts.p, (size_t)n * 4, true));
     *total = tot;
     return GDSJ_OK;
 }
@@ -771,7 +939,7 @@ int gdsj_result_copy(gdsj_handle *h, uint32_t *ids, uint64_t cap) {
         return GDSJ_EOVERFLOW;
     }
     GDSJ_HIP(hipSetDevice(h->device));
-    if (h->total) GDSJ_HIP(hipMemcpy(ids, h->ids.p, (size_t)h->total * 4, hipMemcpyDeviceToHost));
+    GDSJ_TRY(staged_copy(h, ids, h->ids.p, (size_t)h->total * 4, true));
     return GDSJ_OK;
 }
 
@@ -933,11 +1101,12 @@ void copyResultIntoPythonArray(unsigned int *outPythonNeighborTable, unsigned in
     uint64_t m = numResults < g_pending_n ? numResults : g_pending_n;
     if (outPythonNeighborTable && m) {
         gdsj_handle *h = g_pending;
-        hipError_t e = hipSetDevice(h->device);
-        if (e == hipSuccess) e = hipMemcpy(outPythonNeighborTable, h->ids.p, (size_t)m * 4, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
-            set_error("copyResultIntoPythonArray: %s", hipGetErrorString(e));
-            g_legacy_status = GDSJ_EDEVICE;
+        const hipError_t e = hipSetDevice(h->device);
+        const int rc = e == hipSuccess ? staged_copy(h, outPythonNeighborTable, h->ids.p, (size_t)m * 4, true)
+                                       : GDSJ_EDEVICE;
+        if (rc != GDSJ_OK) {
+            if (e != hipSuccess) set_error("copyResultIntoPythonArray: %s", hipGetErrorString(e));
+            g_legacy_status = rc;
         }
     }
     g_pending = nullptr;

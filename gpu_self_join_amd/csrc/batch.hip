@@ -367,6 +367,10 @@ int gdsj_batch_create(int device, const gdsj_instance *inst, uint32_t ninst, int
     }
     gdsj_params p;
     if (pin) p = *pin; else gdsj_params_default(&p);
+    if (p.sort) {
+        set_error("batched joins run without SORT");
+        return GDSJ_EUNSUPPORTED;
+    }
     if (p.filter == GDSJ_FILTER_GRAM || p.filter == GDSJ_FILTER_GRAM_BF16) {
         set_error("batched joins use the plain distance kernels (filter PLAIN or AUTO)");
         return GDSJ_EUNSUPPORTED;

@@ -178,7 +178,7 @@ extern "C" int gdsj_bruteforce_device_noalloc(const void *d_data, uint32_t n, ui
     unsigned char *xs = (unsigned char *)base;
     void *scanTmp = xs + soaBytes;
     hipStream_t s = (hipStream_t)stream;
-    const int mode = acc_mode(p.ilp, p.shortcircuit, p.ilp_fold);
+    const int mode = p.cpu_sum ? ACC_UNFUSED : acc_mode(p.ilp, p.shortcircuit, p.ilp_fold);
     GDSJ_TRY(bruteforce_soa(d_data, n, dims, dtype, ld, xs, s));
     GDSJ_TRY(bruteforce_pass(xs, ld, n, dims, dtype, mode, p.ilp, thr, PASS_COUNT, d_counts, nullptr, nullptr, s));
     GDSJ_TRY(exclusive_scan_u32_u64(d_counts, d_offsets, n, true, scanTmp, s));

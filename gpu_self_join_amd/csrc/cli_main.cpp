@@ -1,9 +1,11 @@
-// cli_main.cpp -- the command-line join: gdsjoin_main <file> <eps> <dim> 3|1
+// cli_main.cpp -- the command-line join: gdsjoin_main <file> <eps> <dim> 3|1|0
 //
 // Drop-in for the reference's `main <dataset> <epsilon> <dims> <mode>`
 // (main.cu:66-120, mode 3 at main.cu:178-331, mode 1 at main.cu:150-175):
 // mode 3 is the grid join, mode 1 the all-pairs GPU join (it prints the
-// total as the reference's brute force does).  Mode 3 imports the text dataset
+// total as the reference's brute force does), mode 0 the table of the
+// reference's CPU brute force (main.cu:121-141: its unfused host sums,
+// computed by the all-pairs kernels, listed point by point).  Mode 3 imports the text dataset
 // (gdsj_csv_load, import_dataset.cpp:49-95), runs the grid self-join on the
 // GPU through the C ABI, prints the time and the total, and appends one line
 // to gpu_stats.txt in the reference's format (main.cu:257-262).  The
@@ -11,6 +13,7 @@
 // at compile time (params.h); here they are options:
 //   --dtype float|double (default float)  --k K (default min(dim, 6))
 //   --ilp I  --shortcircuit 0|1  --ilp-fold 0|1  --reorder 0|1  --stamp 0|1  --linearstamp 0|1
+//   --sort 0|1 (SORT, params.h:46)
 //   --queryreorder 0|1
 //   --estimator exact|learned|sample  --filter plain|gram|gram-bf16|auto  --device D
 //   --print   print every point's sorted neighbor list (PRINTNEIGHBORTABLE
@@ -35,8 +38,9 @@
 static int usage() {
     fprintf(stderr,
             "usage: gdsjoin_main <dataset file> <epsilon> <dims> <mode> [options]\n"
-            "  mode 3: GPU grid self-join; mode 1: all-pairs GPU join (total only)\n"
-            "  options: --dtype float|double --k K --ilp I --shortcircuit 0|1 --ilp-fold 0|1 --reorder 0|1\n"
+            "  mode 3: GPU grid self-join; mode 1: all-pairs GPU join (total only);\n"
+            "  mode 0: the CPU brute force's neighbor table (all pairs, unfused sums)\n"
+            "  options: --dtype float|double --k K --ilp I --shortcircuit 0|1 --ilp-fold 0|1 --reorder 0|1 --sort 0|1\n"
             "           --stamp 0|1 --linearstamp 0|1 --queryreorder 0|1 --estimator exact|learned|sample\n"
             "           --filter plain|gram|gram-bf16|auto --device D --print --table-file\n"
             "           --outlier-scores\n");
@@ -132,6 +136,7 @@ int main(int argc, char **argv) {
         else if (!strcmp(a, "--ilp")) p.ilp = (uint32_t)atoi(v);
         else if (!strcmp(a, "--shortcircuit")) p.shortcircuit = (uint32_t)atoi(v);
         else if (!strcmp(a, "--ilp-fold")) p.ilp_fold = (uint32_t)atoi(v);
+        else if (!strcmp(a, "--sort")) p.sort = (uint32_t)atoi(v);
         else if (!strcmp(a, "--reorder")) p.reorder = (uint32_t)atoi(v);
         else if (!strcmp(a, "--stamp")) p.stamp = atoi(v) ? 1u : (p.stamp == 2 ? 2u : 0u);
         else if (!strcmp(a, "--linearstamp")) p.stamp = atoi(v) ? 2u : (p.stamp == 1 ? 1u : 0u);
@@ -156,9 +161,9 @@ int main(int argc, char **argv) {
         fprintf(stderr, "\nERROR: the number of dimensions must be positive\n");
         return 1;
     }
-    if (mode != 3 && mode != 1) {
+    if (mode != 3 && mode != 1 && mode != 0) {
         fprintf(stderr, "\nERROR: search mode %d is not part of this build (3: GPU grid join, 1: all-pairs "
-                        "GPU join)\n", mode);
+                        "GPU join, 0: the CPU brute force's table)\n", mode);
         return 2;
     }
     // DTYPE epsilon = atof(...): the double value rounded once to DTYPE
@@ -219,6 +224,33 @@ int main(int argc, char **argv) {
         gdsj_destroy(h);
         return 0;
     }
+    if (mode == 0) {
+        // main.cu:121-141: the all-pairs table of CPUBruteForceTable
+        // (main.cu:953-978), whose sums are the host's sequential unfused
+        // runningDist += t*t; computed here by the all-pairs kernels with that
+        // arithmetic, then listed point by point in the reference's format
+        printf("\nBrute force CPU arithmetic (all pairs on the GPU): ");
+        gdsj_params bp = p;
+        bp.cpu_sum = 1;
+        rc = gdsj_bruteforce_host(h, data.data(), (uint32_t)npts, (uint32_t)ndim, dtype, eps, &bp, counts.data(),
+                                  &total);
+        std::vector<uint32_t> ids(total ? total : 1);
+        if (!rc) rc = gdsj_bruteforce_rows_host(h, ids.data(), total);
+        if (rc) {
+            gdsj_destroy(h);
+            return fail("brute force", rc);
+        }
+        printf("\nTotal neighbors in table: %u", (unsigned)total);
+        uint64_t off = 0;
+        for (uint64_t i = 0; i < npts; i++) {
+            printf("\npoint id: %d, neighbors: ", (int)i);
+            for (uint32_t j = 0; j < counts[i]; j++) printf("%d,", (int)ids[off + j]);
+            off += counts[i];
+        }
+        printf("\n");
+        gdsj_destroy(h);
+        return 0;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     rc = gdsj_join_host(h, data.data(), (uint32_t)npts, (uint32_t)ndim, dtype, eps, &p, counts.data(), &total);
     const auto t1 = std::chrono::steady_clock::now();
@@ -238,7 +270,7 @@ int main(int argc, char **argv) {
     std::ofstream stats("gpu_stats.txt", std::ios::app);
     stats << secs << ", " << fname << ", " << eps << ", " << total
           << ", GPUNUMDIM/NUMINDEXEDDIM/ILP/STAMP/SORT/REORDER/SHORTCIRCUIT/QUERYREORDER/DTYPE(float/double): "
-          << ndim << ", " << k << ", " << p.ilp << ", " << (p.stamp == 1 ? 1 : 0) << ", " << 0 << ", " << p.reorder << ", "
+          << ndim << ", " << k << ", " << p.ilp << ", " << (p.stamp == 1 ? 1 : 0) << ", " << p.sort << ", " << p.reorder << ", "
           << p.shortcircuit << ", " << p.queryreorder << ", " << (dtype == GDSJ_DTYPE_F64 ? "double" : "float")
           << std::endl;
     stats.close();

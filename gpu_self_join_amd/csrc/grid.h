@@ -59,6 +59,11 @@ struct Grid {
     std::vector<uint4> hComboTab;            //   its host copy (source of the upload)
     uint64_t runStride = 0;
     bool haveRuns = false;
+    // SORT (params.h:46; set before grid_build): each cell's points ordered by
+    // their value at dim 0 when every dim is indexed, else at dim k (after
+    // REORDER); posCell maps a sorted position to its cell
+    int sortCells = 0;
+    DevBuf posCell;
     DevBuf keyA, keyB, valB, tmp, aux, flag, tileKey, tileVal, cellAux, scalars;
     HostBuf hostSample;  // REORDER's sample rows, pinned (the variance is summed on the host)
     void release();

@@ -116,12 +116,35 @@ __global__ __launch_bounds__(256) void k_minmax(const T *__restrict__ data, uint
 }
 
 // ----------------------------------------------------------- cell keys --
+// Order-preserving integer key of a DTYPE value (-0 and +0 equal)
+__device__ __forceinline__ uint64_t value_order_key(float v) {
+    const uint32_t u = __float_as_uint(v == 0.0f ? 0.0f : v);
+    return (u & 0x80000000u) ? (uint64_t)(~u) : (uint64_t)(u | 0x80000000u);
+}
+__device__ __forceinline__ uint64_t value_order_key(double v) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v == 0.0 ? 0.0 : v);
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+
+// SORT: keys = the points' values at column col, vals = point ids
+template <typename T>
+__global__ void k_value_keys(const T *__restrict__ data, uint32_t n, uint32_t d, uint32_t col,
+                             uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = value_order_key(data[(uint64_t)i * d + col]);
+    vals[i] = i;
+}
+
+// cell linear ids of the points order[0 .. n) (order nullable: the
+// identity); vals[i] = the point of slot i (vals may alias order)
 template <typename T>
 __global__ void k_cell_keys(const T *__restrict__ data, uint32_t n, uint32_t d, KeyArgs a, T eps,
-                            uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                            uint32_t *__restrict__ outside) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+                            uint64_t *__restrict__ keys, uint32_t *vals,
+                            uint32_t *__restrict__ outside, const uint32_t *order) {
+    uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= n) return;
+    const uint32_t i = order ? order[slot] : slot;
     uint64_t lin = 0;
     bool bad = false;
     for (uint32_t j = 0; j < a.k; j++) {
@@ -131,9 +154,17 @@ __global__ void k_cell_keys(const T *__restrict__ data, uint32_t n, uint32_t d, 
         bad |= !(q < (T)a.nCells[j]);      // only when eps is below the data's precision
         lin += (uint64_t)c * a.mult[j];
     }
-    keys[i] = lin;
-    vals[i] = i;
+    keys[slot] = lin;
+    vals[slot] = i;
     if (bad) atomicOr(outside, 1u);
+}
+
+// SORT: the cell of every sorted position (inclusive count of cell heads - 1)
+__global__ void k_pos_cell(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ cidx, uint32_t n,
+                           uint32_t *__restrict__ posCell) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    posCell[i] = cidx[i] + flag[i] - 1u;
 }
 
 __global__ void k_heads(const uint64_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ flag) {
@@ -308,7 +339,7 @@ __global__ void k_tiles_permute(const uint4 *__restrict__ in, const uint32_t *__
 void Grid::release() {
     DevBuf *b[] = {&xs, &ids, &cellKey, &cellStart, &cellCoord, &hkeys, &hvals, &tiles, &keyA, &keyB,
                    &valB, &tmp, &aux, &flag, &tileKey, &tileVal, &cellAux, &scalars, &runs, &nruns, &cellCand, &cellHist,
-                   &comboTab};
+                   &comboTab, &posCell};
     for (DevBuf *p : b) p->release();
     hostSample.release();
 }
@@ -426,25 +457,44 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     uint32_t *vA = G.ids.as<uint32_t>(), *vB = G.valB.as<uint32_t>();
     uint32_t *outside = G.scalars.as<uint32_t>() + 1;
     GDSJ_HIP(hipMemsetAsync(outside, 0, 4, s));
-    k_cell_keys<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, a, (T)G.eps, kA, vA, outside);
+    uint32_t *v0 = vA, *v1 = vB;  // the value buffer the cell sort starts from, and its alternate
+    if (G.sortCells) {
+        // SORT (GPU.cu:351-391): a stable sort by the value at the sort dim
+        // first, so that the stable cell sort leaves each cell in value order
+        const uint32_t col = G.perm[k == d ? 0 : k];
+        k_value_keys<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, col, kA, vA);
+        int altv = 0;
+        GDSJ_TRY(radix_sort_pairs(kA, vA, kB, vB, n, (int)(8 * sizeof(T)), G.tmp.p, s, &altv));
+        if (altv) {
+            v0 = vB;
+            v1 = vA;
+        }
+        k_cell_keys<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, a, (T)G.eps, kA, v0, outside, v0);
+    } else {
+        k_cell_keys<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, a, (T)G.eps, kA, v0, outside, nullptr);
+    }
     int alt = 0;
     int bits = bit_length(G.totalCells - 1);
-    GDSJ_TRY(radix_sort_pairs(kA, vA, kB, vB, n, bits, G.tmp.p, s, &alt));
+    GDSJ_TRY(radix_sort_pairs(kA, v0, kB, v1, n, bits, G.tmp.p, s, &alt));
     uint64_t *skeys = alt ? kB : kA;
-    uint32_t *svals = alt ? vB : vA;
+    uint32_t *svals = alt ? v1 : v0;
     // ---- cells
     GDSJ_TRY(G.flag.ensure(((size_t)n + 1) * 4 * 2 + 64));
     uint32_t *flag = G.flag.as<uint32_t>();
     uint32_t *cidx = flag + n + 1;
     k_heads<<<ceil_div(n, 256), 256, 0, s>>>(skeys, n, flag);
     GDSJ_TRY(exclusive_scan_u32_u32(flag, cidx, n, true, G.tmp.p, s));
+    if (G.sortCells) {
+        GDSJ_TRY(G.posCell.ensure((size_t)n * 4 + 256));
+        k_pos_cell<<<ceil_div(n, 256), 256, 0, s>>>(flag, cidx, n, G.posCell.as<uint32_t>());
+    }
     // ---- SoA (sorted order, dims permuted), launched before the cell count sync
     G.ld = ((uint64_t)n + 1 + 63) / 64 * 64;  // n points + NaN padding
     GDSJ_TRY(G.xs.ensure((size_t)G.ld * d * sizeof(T) + 256));
     uint32_t *dperm = (uint32_t *)((char *)G.aux.p + (((size_t)mmBlocks * k * 2 * sizeof(T) + 255) & ~(size_t)255));
     GDSJ_HIP(hipMemcpyAsync(dperm, G.perm, d * 4, hipMemcpyHostToDevice, s));
     uint32_t *nonfinite = G.scalars.as<uint32_t>();
-    uint32_t *idsOut = alt ? vA : vB;  // the value buffer the sort did not end in
+    uint32_t *idsOut = svals == vA ? vB : vA;  // the value buffer the sort did not end in
     GDSJ_HIP(hipMemsetAsync(nonfinite, 0, 4, s));
     if ((size_t)d * 65 * sizeof(T) <= 48 * 1024)
         k_soa_tile<T, 64><<<ceil_div(n, 64), 256, (size_t)d * 65 * sizeof(T), s>>>(data, n, d, dperm, svals,

@@ -10,6 +10,8 @@ enum AccMode : int {
     ACC_SEQ = 0,       // ILP == 0: one FMA accumulator, dims in order
     ACC_ILP_SC = 1,    // ILP > 0, SHORTCIRCUIT == 1: fold after every ILP dims
     ACC_ILP_NOSC = 2,  // ILP > 0, SHORTCIRCUIT == 0: ILP partials, fold at end
+    ACC_UNFUSED = 3,   // the reference's CPU paths: acc = acc + round(t*t), dims in order
+                       // (main.cu:963-967; all-pairs join only)
 };
 
 // The accumulation a join runs for ILP, SHORTCIRCUIT and the fold reading
@@ -49,6 +51,11 @@ struct JoinArgs {
     // (gram_valu_band).  gv = 0: the reference's arithmetic for every pair.
     uint32_t gv;
     float gvLo, gvMid, gvHalf;  // band [gvLo, gvLo + 2 half'], gvHalf >= half' with room
+    // SORT (kernel.cu:400-467; generic kernels): 1 every dim indexed (sort
+    // dim 0), 2 the first unindexed dim sortDim; cells in value order,
+    // posCell = cell of each sorted position
+    uint32_t sortMode, sortDim;
+    const uint32_t *posCell;
     uint32_t d;        // dims
     uint32_t ilp;      // ILP
     uint32_t sc;       // SHORTCIRCUIT early exit

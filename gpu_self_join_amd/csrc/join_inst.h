@@ -32,6 +32,27 @@ static int launch_join_t(const Grid &G, const JoinArgs &a, int pass, hipStream_t
     return GDSJ_OK;
 }
 
+// SORT (JoinArgs::sortMode): the generic kernels, counting, fill and
+// slotted passes (no unicomp, no hit masks)
+template <int MODE>
+static int launch_join_sort(const JoinArgs &a, int pass, hipStream_t s) {
+    const uint32_t blocks = (a.ntiles + 3) / 4;
+    if (blocks == 0) return GDSJ_OK;
+    if (a.stamp || a.hmask) {
+        set_error("SORT runs without unicomp and without hit masks");
+        return GDSJ_EINVAL;
+    }
+    const JT *xs = (const JT *)a.xs;
+    if (pass == PASS_FILL)
+        k_join<JT, 0, MODE, 0, PASS_FILL, false, false, false, true><<<blocks, 256, 0, s>>>(xs, a.ids, a, nullptr);
+    else if (pass == PASS_SLOT)
+        k_join<JT, 0, MODE, 0, PASS_SLOT, false, false, false, true><<<blocks, 256, 0, s>>>(xs, a.ids, a, nullptr);
+    else
+        k_join<JT, 0, MODE, 0, PASS_COUNT, false, false, false, true><<<blocks, 256, 0, s>>>(xs, a.ids, a, nullptr);
+    GDSJ_HIP(hipGetLastError());
+    return GDSJ_OK;
+}
+
 // the Gram-form filter variants exist for FP32 and the dims JOIN_GV_DIM accepts
 template <int D, int MODE, int ILPV>
 static int launch_join_gv(const Grid &G, const JoinArgs &a, int pass, hipStream_t s) {
@@ -48,6 +69,11 @@ static int launch_join_gv(const Grid &G, const JoinArgs &a, int pass, hipStream_
         break;
 
 int JNAME(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t s) {
+    if (a.sortMode) {
+        if (mode == ACC_SEQ) return launch_join_sort<ACC_SEQ>(a, pass, s);
+        if (mode == ACC_ILP_SC) return launch_join_sort<ACC_ILP_SC>(a, pass, s);
+        return launch_join_sort<ACC_ILP_NOSC>(a, pass, s);
+    }
     switch (a.d) {
         JOIN_DIMS
         default:

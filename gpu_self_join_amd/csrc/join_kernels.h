@@ -104,6 +104,15 @@ __device__ __forceinline__ T dist_reg(const T (&q)[D], const T (&c)[D], T thr, b
 template <typename T>
 __device__ __forceinline__ T dist_mem(const T *__restrict__ xs, uint64_t ld, uint32_t qpos, uint32_t cpos,
                                       uint32_t d, int mode, uint32_t ilp) {
+    if (mode == ACC_UNFUSED) {
+        T acc = 0;
+        for (uint32_t l = 0; l < d; l++) {
+            T t = xs[l * ld + cpos] - xs[l * ld + qpos];
+            T p = t * t;
+            acc = acc + p;
+        }
+        return acc;
+    }
     if (mode == ACC_SEQ) {
         T acc = 0;
         for (uint32_t l = 0; l < d; l++) {
@@ -235,6 +244,51 @@ __device__ __forceinline__ T dist_row(const T *__restrict__ qrow, const T *__res
     return acc;
 }
 
+// SORT=1 (kernel.cu:400-467): whether the reference evaluates candidate c
+// (sorted position cpos, in cell ccell) for query q (position qpos) at all.
+// Its cells are ordered by the value at the sort dim.
+// sortMode 1 (every dim indexed, sort dim 0, kernel.cu:403-431): a cell with
+// the query cell's dim-0 coordinate qc0 is walked whole; a cell on either
+// side is walked from the query's side and the walk stops at the first value
+// with |value - query| > eps.  The values on one side lie all below (above)
+// the query's, so |value - query| grows along the walk and the walked points
+// are those with |value - query| <= eps.
+// sortMode 2 (sort dim k, kernel.cu:433-465): the reference's stepping
+// search gives a start index; the points from it on are walked while
+// value - query < eps, which, the values ascending, is every point at or
+// past the start with value - query < eps.
+template <typename T>
+__device__ __forceinline__ bool sort_evaluated(const T *__restrict__ xs, uint64_t ld, const JoinArgs &a,
+                                               uint32_t qpos, uint32_t cpos, uint32_t ccell, uint32_t qc0) {
+    const T eps = (T)a.ga.eps;
+    if (a.sortMode == 1) {
+        if (a.cellCoord[ccell] == qc0) return true;
+        const T diff = xs[cpos] - xs[qpos];
+        return (diff < (T)0 ? -diff : diff) <= eps;
+    }
+    const T *col = xs + (uint64_t)a.sortDim * ld;
+    const uint32_t s = a.cellStart[ccell];
+    const int length = (int)(a.cellStart[ccell + 1] - s) - 1;
+    const T q = col[qpos];
+    int idx = length / 2;
+    for (int step = (length + 3) / 4; step > 1; step = (step + 1) / 2) {
+        const T dq = q - col[s + (uint32_t)idx];
+        if (dq >= eps) {
+            idx += step;
+            if (idx > length) idx = length;
+        } else {
+            idx -= step;
+            if (idx < 0) {
+                idx = 0;
+                step = 0;
+            }
+        }
+    }
+    if (idx > 0 && q - col[s + (uint32_t)idx - 1] <= eps) idx--;
+    if (idx > 0 && q - col[s + (uint32_t)idx - 1] <= eps) idx--;
+    return (int)(cpos - s) >= idx && col[cpos] - q < eps;
+}
+
 // One work tile of the join (one wave): the body shared by the per-join
 // kernel below and the persistent batched kernel (batch.hip).
 // CM: candidate chunks held in VGPRs (0: CMaxFor's default).
@@ -256,7 +310,7 @@ __device__ __forceinline__ T dist_row(const T *__restrict__ qrow, const T *__res
 // hits from them and loads no coordinates.  Both passes cut the candidates
 // into the same windows and chunk batches, so chunk numbers agree.
 template <typename T, int D, int MODE, int ILPV, int PASS, int CM = 0, bool STAMP = false, int QRW = 16,
-          bool UNSW = true, bool BMAP = true, bool HM = false, bool GV = false>
+          bool UNSW = true, bool BMAP = true, bool HM = false, bool GV = false, bool SRT = false>
 __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
                                           const JoinArgs &a, const uint32_t tile,
                                           const uint64_t *__restrict__ hmr = nullptr) {
@@ -294,6 +348,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
         }
     }
     const uint32_t qid_l = lane < nq ? ids[q0 + lane] : 0u;
+    const uint32_t sqc0 = SRT ? uniform(a.cellCoord[cell]) : 0u;  // SORT: the query cell's dim-0 coordinate
     __shared__ uint32_t s_qid[STAMP ? 256 / GDSJ_WAVE : 1][GDSJ_WAVE];
     constexpr int BMW = 64;  // run-start bitmap words per wave
     __shared__ uint64_t s_bm[BMAP ? 256 / GDSJ_WAVE : 1][BMAP ? BMW : 1];
@@ -405,6 +460,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
             T c[CMAX][D > 0 ? D : 1];
             uint32_t cpos[CMAX];
             uint32_t cid[CMAX];
+            uint32_t scell[SRT ? CMAX : 1];  // SORT: the candidate's cell
             T cn[GVF ? CMAX : 1];  // GV: |c'|^2 of this lane's candidate
             bool valid[CMAX];
             bool other[CMAX];                                    // STAMP: candidate outside the own cell
@@ -449,6 +505,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                     }
                 }
                 if (NEED_IDS) cid[ch] = (ch < (int)nch) ? ids[sp] : 0u;
+                if (SRT) scell[ch] = (ch < (int)nch) ? a.posCell[sp] : 0u;
             }
             // unswitched on the fill pass's capacity check where it pays
             // (the single-join kernel); one copy with the check otherwise
@@ -516,8 +573,10 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
 }
 
 // hm: the hit masks (HM; a.hmask), read-only here in the fill pass
-// GV: the Gram-form filter (JoinArgs::gv) decides the pairs outside its band
-template <typename T, int D, int MODE, int ILPV, int PASS, bool STAMP = false, bool HM = false, bool GV = false>
+// GV: the Gram-form filter (JoinArgs::gv) decides the pairs outside its band;
+// SRT: SORT's candidate windows (generic kernels only)
+template <typename T, int D, int MODE, int ILPV, int PASS, bool STAMP = false, bool HM = false, bool GV = false,
+          bool SRT = false>
 __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
                                               const JoinArgs a, const uint64_t *__restrict__ hm) {
     const uint32_t lblock = a.xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -529,7 +588,7 @@ __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const ui
     // fill from its masks cut the candidates into the same chunks
     constexpr int CM = PASS == PASS_SLOT ? CMaxFor<T, GV ? D + 1 : D, GDSJ_CMAX_WORDS_SLOT>::value
                                          : (GV && !HM ? CMaxFor<T, D + 1>::value : 0);
-    join_tile<T, D, MODE, ILPV, PASS, CM, STAMP, 16, true, true, HM, GV>(xs, ids, a, tile, hm);
+    join_tile<T, D, MODE, ILPV, PASS, CM, STAMP, 16, true, true, HM, GV, SRT>(xs, ids, a, tile, hm);
 }
 
 }  // namespace gdsj

@@ -95,6 +95,16 @@ typedef struct gdsj_params {
                                in tile-local coordinates, and those within a proven error
                                band of the threshold by the reference's arithmetic: 0 auto
                                (off), 1 on where it applies, 2 off (speed only) */
+    uint32_t sort;          /* SORT (params.h:46): each cell's points ordered by their value at
+                               dim 0 when every dim is indexed, else at dim k after REORDER
+                               ("sortidu"), and each query walks only the part of a neighbor
+                               cell the reference walks (kernel.cu:400-467).  Changes results
+                               at the epsilon boundary of the sort dim as the reference does.
+                               Runs with the plain kernels, without unicomp. 0 off, 1 on */
+    uint32_t cpu_sum;       /* all-pairs join only: 1 sums the squared differences as the
+                               reference's CPU paths do, sequentially and unfused
+                               (runningDist += t*t, main.cu:963-967; the CLI's mode 0), in
+                               place of the kernels' reading of ILP / SHORTCIRCUIT */
 } gdsj_params;
 
 /* Timing and diagnostics of the last join on a handle. */
@@ -192,6 +202,11 @@ int gdsj_bruteforce_device(gdsj_handle *h, const void *d_data, uint32_t n, uint3
 int gdsj_bruteforce_host(gdsj_handle *h, const void *data, uint32_t n, uint32_t dims,
                          int dtype, double epsilon, const gdsj_params *p,
                          uint32_t *counts, uint64_t *total);
+/* The rows of the last gdsj_bruteforce_host on h (each ascending, in point
+ * order) into the host buffer ids[cap], cap >= its total: the neighbor table
+ * of the reference's CPU brute force (CPUBruteForceTable main.cu:953-978,
+ * printed by search mode 0, main.cu:121-141). */
+int gdsj_bruteforce_rows_host(gdsj_handle *h, uint32_t *ids, uint64_t cap);
 
 /* Allocation-free all-pairs join.  No counterpart in the reference: it is
  * for callers that cannot allocate device memory (code under HIP graph

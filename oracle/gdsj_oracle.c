@@ -49,6 +49,7 @@ typedef struct {
     uint32_t reorder;      /* REORDER                                         */
     uint32_t fold_fused;   /* ILP>0 && SHORTCIRCUIT: 1 = fused fold reading   */
     int32_t nthreads;      /* OpenMP threads, 0 = default                     */
+    uint32_t sort;         /* SORT (params.h:46): cells ordered by one dim    */
 } orc_params;
 
 /* order-independent per-point hash term (splitmix64 finalizer) */

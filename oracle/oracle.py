@@ -16,7 +16,8 @@ _LIB = None
 
 class OrcParams(C.Structure):
     _fields_ = [("k", C.c_uint32), ("ilp", C.c_uint32), ("shortcircuit", C.c_uint32),
-                ("reorder", C.c_uint32), ("fold_fused", C.c_uint32), ("nthreads", C.c_int32)]
+                ("reorder", C.c_uint32), ("fold_fused", C.c_uint32), ("nthreads", C.c_int32),
+                ("sort", C.c_uint32)]
 
 
 def build():
@@ -60,8 +61,8 @@ def _sfx(dt):
     return "f32" if np.dtype(dt) == np.float32 else "f64"
 
 
-def _params(k, ilp=0, shortcircuit=0, reorder=0, fold_fused=0, nthreads=0):
-    return OrcParams(k, ilp, shortcircuit, reorder, fold_fused, nthreads)
+def _params(k, ilp=0, shortcircuit=0, reorder=0, fold_fused=0, nthreads=0, sort=0):
+    return OrcParams(k, ilp, shortcircuit, reorder, fold_fused, nthreads, sort)
 
 
 def _ptr(a):
@@ -78,11 +79,11 @@ def _scalar(dt, eps):
     return C.c_float(eps) if np.dtype(dt) == np.float32 else C.c_double(eps)
 
 
-def join(data, eps, k, ilp=0, shortcircuit=0, reorder=0, fold_fused=0, with_ids=True, nthreads=0):
+def join(data, eps, k, ilp=0, shortcircuit=0, reorder=0, fold_fused=0, with_ids=True, nthreads=0, sort=0):
     """Full self-join on the CPU.  Returns dict(counts, hashes, ids|None, stats)."""
     data = _prep(data)
     n, d = data.shape
-    p = _params(k, ilp, shortcircuit, reorder, fold_fused, nthreads)
+    p = _params(k, ilp, shortcircuit, reorder, fold_fused, nthreads, sort)
     counts = np.zeros(n, np.uint32)
     hashes = np.zeros(n, np.uint64)
     stats = np.zeros(8, np.uint64)

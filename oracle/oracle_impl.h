@@ -181,12 +181,15 @@ typedef struct {
     uint32_t ncell;
 } SFX(orc_grid_t);
 
-typedef struct { uint64_t key; uint32_t id; } SFX(orc_kv_t);
+typedef struct { uint64_t key; T val; uint32_t id; } SFX(orc_kv_t);
 
+/* cell linear id, then (SORT only; val = 0 otherwise) the value at the sort
+ * dim, then point id (std::sort in GPU.cu:378 leaves ties unspecified) */
 static int SFX(orc_kv_cmp)(const void *a, const void *b)
 {
     const SFX(orc_kv_t) *x = (const SFX(orc_kv_t) *)a, *y = (const SFX(orc_kv_t) *)b;
     if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    if (x->val != y->val) return x->val < y->val ? -1 : 1;
     return x->id < y->id ? -1 : (x->id > y->id);
 }
 
@@ -252,6 +255,10 @@ static int SFX(orc_grid_build)(SFX(orc_grid_t) *g, const T *data, uint32_t n,
         }
         kv[i].key = lin;
         kv[i].id = (uint32_t)i;
+        /* SORT (GPU.cu:351-391): a cell's points ordered by their value at
+         * dim 0 when every dim is indexed, else at the first unindexed dim
+         * (after REORDER) */
+        kv[i].val = p->sort ? g->pts[i * d + (k == d ? 0 : k)] : (T)0;
     }
     qsort(kv, n, sizeof(SFX(orc_kv_t)), SFX(orc_kv_cmp));
     g->order = (uint32_t *)malloc(sizeof(uint32_t) * n);
@@ -342,6 +349,73 @@ static int SFX(orc_u32_cmp)(const void *a, const void *b)
     return x < y ? -1 : (x > y);
 }
 
+/* SORT=1 (kernel.cu:400-467): the part [*s, *e) of candidate cell c, whose
+ * points are ordered by their value at the sort dim, that the reference
+ * evaluates for query qi.
+ * Every dim indexed (sort dim 0, kernel.cu:403-431): a cell with query qi's
+ * coordinate in dim 0 is evaluated whole; a cell to the left is walked from
+ * its last point down, one to the right from its first point up, and the
+ * walk stops at the first point whose |value - query value| exceeds eps.
+ * Otherwise (sort dim k, "sortidu", kernel.cu:433-465): a stepping search
+ * for the first point whose value is within eps below the query's, two steps
+ * back while still within eps, then points while value - query < eps. */
+static void SFX(orc_sort_window)(const SFX(orc_grid_t) *g, int64_t c, uint32_t qi, uint32_t *s, uint32_t *e)
+{
+    const uint32_t lo = *s, hi = *e;
+    const T eps = g->eps;
+    if (g->k == g->d) {
+        const T q = g->pts[(size_t)qi * g->d];
+        const uint32_t qc = g->cellOf[(size_t)qi * g->k];
+        const uint32_t cc = (uint32_t)(g->cellId[c] % g->nCells[0]);
+        if (cc == qc) return;
+        if (qc > cc) { /* left: from the last point down */
+            uint32_t t = hi;
+            while (t > lo) {
+                const T v = g->spts[(size_t)(t - 1) * g->d];
+                const T diff = v - q;
+                if ((diff < 0 ? -diff : diff) > eps) break;
+                t--;
+            }
+            *s = t;
+        } else {
+            uint32_t t = lo;
+            while (t < hi) {
+                const T v = g->spts[(size_t)t * g->d];
+                const T diff = v - q;
+                if ((diff < 0 ? -diff : diff) > eps) break;
+                t++;
+            }
+            *e = t;
+        }
+        return;
+    }
+    const uint32_t sd = g->k;
+    const T q = g->pts[(size_t)qi * g->d + sd];
+#define ORC_V(i) (g->spts[(size_t)(lo + (uint32_t)(i)) * g->d + sd])
+    const int length = (int)(hi - lo) - 1;
+    int idx = length / 2;
+    for (int step = (length + 3) / 4; step > 1; step = (step + 1) / 2) {
+        const T dq = q - ORC_V(idx);
+        if (dq >= eps) {
+            idx += step;
+            if (idx > length) idx = length;
+        } else {
+            idx -= step;
+            if (idx < 0) {
+                idx = 0;
+                step = 0;
+            }
+        }
+    }
+    if (idx > 0 && q - ORC_V(idx - 1) <= eps) idx--;
+    if (idx > 0 && q - ORC_V(idx - 1) <= eps) idx--;
+    int end = idx;
+    while (end <= length && ORC_V(end) - q < eps) end++;
+#undef ORC_V
+    *s = lo + (uint32_t)idx;
+    *e = lo + (uint32_t)end;
+}
+
 /* core: evaluates query point qi against every candidate of its adjacent
  * cells.  Returns the count; optionally appends ids and accumulates stats. */
 ORC_HOT static uint32_t SFX(orc_query)(const SFX(orc_grid_t) *g, const int64_t *adj,
@@ -361,6 +435,7 @@ ORC_HOT static uint32_t SFX(orc_query)(const SFX(orc_grid_t) *g, const int64_t *
     const T stop = e2 + band;
     for (uint32_t a = 0; a < nadj; a++) {
         uint32_t s = g->cellStart[adj[a]], e = g->cellStart[adj[a] + 1];
+        if (p->sort) SFX(orc_sort_window)(g, adj[a], qi, &s, &e);
         cand += e - s;
         for (uint32_t t = s; t < e; t++) {
             uint32_t cj = g->order[t];

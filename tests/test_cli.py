@@ -71,7 +71,7 @@ def test_csv_missing_file_is_an_error(tmp_path):
 def test_cli_usage_errors():
     r = subprocess.run([CLI], capture_output=True, text=True)
     assert r.returncode != 0 and "Incorrect number of input parameters" in r.stderr
-    r = subprocess.run([CLI, "x.txt", "1.0", "2", "0"], capture_output=True, text=True)
+    r = subprocess.run([CLI, "x.txt", "1.0", "2", "9"], capture_output=True, text=True)
     assert r.returncode != 0 and "not part of this build" in r.stderr
 
 
@@ -161,3 +161,39 @@ def test_cli_mode1_bruteforce_total(gpu, tmp_path):
     xs = _lib.csv_load(str(p)).reshape(-1, 2)
     ref = O.bruteforce(xs, float(np.float32(0.75)), ilp=0, shortcircuit=0, reorder=0)
     assert m and int(m.group(1)) == int(ref["counts"].sum())
+
+
+@pytest.mark.gpu
+def test_cli_mode0_cpu_bruteforce_table(gpu, tmp_path):
+    """Mode 0 (main.cu:121-141): the table of the reference's CPU brute
+    force, whose sums are sequential and unfused (main.cu:963-967), listed
+    point by point; checked against a float32 numpy restatement of those sums
+    and against the restated CPU brute force's counts."""
+    rng = np.random.default_rng(17)
+    x = (rng.random((1500, 3)) * 6.0).astype(np.float32)
+    path = tmp_path / "pts.txt"
+    path.write_text("\n".join(",".join("%.9g" % v for v in row) for row in x) + "\n")
+    xs = _lib.csv_load(str(path)).reshape(-1, 3)
+    eps = np.float32(0.7)
+    r = subprocess.run([CLI, str(path), "0.7", "3", "0"], capture_output=True, text=True, cwd=str(tmp_path),
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    # the CPU brute force's arithmetic: t = p_i - p_j, acc = acc + round(t*t)
+    want = []
+    for i in range(len(xs)):
+        acc = np.zeros(len(xs), np.float32)
+        for k in range(3):
+            t = xs[i, k] - xs[:, k]
+            acc = acc + t * t
+        want.append(np.nonzero(np.sqrt(acc) <= eps)[0])
+    total = sum(len(w) for w in want)
+    assert "Total neighbors in table: %d" % total in r.stdout
+    rows = [ln for ln in r.stdout.splitlines() if ln.startswith("point id: ")]
+    assert len(rows) == len(xs)
+    for i, ln in enumerate(rows):
+        head, tail = ln.split("neighbors: ")
+        assert head == "point id: %d, " % i
+        got = np.array([int(v) for v in tail.split(",") if v], dtype=np.int64)
+        assert np.array_equal(got, want[i]), i
+    c, _ = O.cpu_bruteforce(xs, float(eps), np.arange(len(xs), dtype=np.uint32), 4)
+    assert np.array_equal(c, np.array([len(w) for w in want]))

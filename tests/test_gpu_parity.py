@@ -389,3 +389,27 @@ def test_vfilter_on_off_same_table(sj, d):
                 c, ids, r = _gpu_join(sj, data, e, k=d, reorder=0, vfilter=vf, ilp=8, shortcircuit=1, estimator=est)
             ref = O.join(data, e, k=d, reorder=0, ilp=8, shortcircuit=1)
             assert_csr_equal(c, ids, ref["counts"], ref["ids"])
+
+
+@pytest.mark.parametrize("d,k,dt", [(3, 3, np.float32), (6, 6, np.float32), (3, 2, np.float32),
+                                    (8, 4, np.float32), (5, 3, np.float64), (4, 4, np.float64)])
+def test_sort_matches_oracle(sj, d, k, dt):
+    """SORT (sort dim 0 when every dim is indexed, else dim k): the GPU's
+    table equals the oracle's on clustered data and on an integer lattice
+    whose epsilon-boundary pairs SORT treats differently from the plain join,
+    with the exact passes and with learned slots."""
+    rng = np.random.default_rng(300 + d + k)
+    centers = rng.random((6, d)) * 10
+    x = (centers[rng.integers(0, 6, 4000)] + rng.standard_normal((4000, d))).astype(dt)
+    eps = _eps_for(x, 30)
+    side = {3: 9, 4: 6, 5: 4, 6: 3, 8: 3}[d]
+    lat = (np.stack(np.meshgrid(*[np.arange(side, dtype=dt)] * d, indexing="ij"), -1).reshape(-1, d) + dt(10))
+    for data, e, reorder in ((x, eps, 1), (np.ascontiguousarray(lat), 1.0, 0)):
+        ref = O.join(data, e, k=k, ilp=8, shortcircuit=1, reorder=reorder, sort=1)
+        c, ids, r = _gpu_join(sj, data, e, k=k, ilp=8, shortcircuit=1, reorder=reorder, sort=1, stamp=0)
+        print("sort d=%d k=%d n=%d total=%d" % (d, k, len(data), r.total), flush=True)
+        assert_csr_equal(c, ids, ref["counts"], ref["ids"])
+        for est in (_lib.EST_LEARNED, _lib.EST_LEARNED):  # train, then the slotted pass
+            c, ids, r = _gpu_join(sj, data, e, k=k, ilp=8, shortcircuit=1, reorder=reorder, sort=1, stamp=0,
+                                  estimator=est)
+        assert_csr_equal(c, ids, ref["counts"], ref["ids"])

@@ -238,3 +238,42 @@ def test_fold_ambiguity_on_golden_fixtures():
         print(os.path.basename(path), "ambiguous pairs", len(diff))
         seen += 1
     assert seen >= 3
+
+
+# ---- SORT (kernel.cu:400-467) --------------------------------------------
+def _lattice(d, side=7, base=10.0):
+    g = np.stack(np.meshgrid(*[np.arange(side, dtype=np.float32)] * d, indexing="ij"), -1).reshape(-1, d)
+    return np.ascontiguousarray(g + np.float32(base))
+
+
+def test_sort_equals_plain_join_away_from_the_boundary():
+    """On random data SORT only skips candidates that fail the predicate: the
+    tables equal the join without SORT, with fewer candidates evaluated."""
+    rng = np.random.default_rng(5)
+    for d, k in ((3, 3), (6, 6), (8, 4), (5, 2)):
+        c = rng.random((6, d)) * 10
+        x = (c[rng.integers(0, 6, 3000)] + rng.standard_normal((3000, d))).astype(np.float32)
+        a = O.join(x, 0.8, k=k, ilp=8, shortcircuit=1, reorder=1)
+        b = O.join(x, 0.8, k=k, ilp=8, shortcircuit=1, reorder=1, sort=1)
+        assert np.array_equal(a["counts"], b["counts"]) and np.array_equal(a["ids"], b["ids"])
+        assert b["stats"][1] < a["stats"][1]
+
+
+def test_sortidu_window_ends_strictly_below_eps():
+    """With a sort dim beyond the indexed ones, the walk runs while value -
+    query < eps (kernel.cu:455): on an integer lattice with eps = 1 exactly
+    the pairs (q, q + e_sd) drop out, and (q + e_sd, q) stays (the search
+    steps back while query - value <= eps)."""
+    x = _lattice(3)
+    a = O.join(x, 1.0, k=2, reorder=0)
+    b = O.join(x, 1.0, k=2, reorder=0, sort=1)
+    n = len(x)
+    rows = np.repeat(np.arange(n), a["counts"].astype(np.int64))
+    keep = ~((x[a["ids"], 2] - x[rows, 2] == 1.0) & np.all(x[a["ids"], :2] == x[rows, :2], axis=1))
+    want = np.bincount(rows[keep], minlength=n)
+    assert np.array_equal(b["counts"], want)
+    assert int(a["counts"].sum() - b["counts"].sum()) == 294  # 7 x 7 columns x 6 steps
+    # every dim indexed (sort dim 0): the walk stops only past eps, so the
+    # lattice keeps every pair
+    c = O.join(x, 1.0, k=3, reorder=0, sort=1)
+    assert np.array_equal(c["counts"], O.join(x, 1.0, k=3, reorder=0)["counts"])
