@@ -65,6 +65,31 @@ __device__ __forceinline__ uint64_t prefix_mask(uint32_t n) {
     return n >= (uint32_t)GDSJ_WAVE ? ~0ull : ((1ull << n) - 1ull);
 }
 
+// element at byte offset bo (below 2^32) from a wave-uniform base: the
+// access takes the base in scalar registers and a 32-bit offset per lane
+// instead of a 64-bit address per lane
+template <typename T>
+__device__ __forceinline__ T load_at(const T *__restrict__ base, uint32_t bo) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + bo);
+}
+template <typename T>
+__device__ __forceinline__ void store_at(T *__restrict__ base, uint32_t bo, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + bo) = v;
+}
+
+// raw buffer resource over [base, base + 4 GiB): a load takes it in scalar
+// registers with a scalar and a 32-bit per-lane byte offset (third dword:
+// 32-bit data format, the gfx9 family's raw-buffer setting)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)0xFFFFFFFF, 0x00020000);
+}
+__device__ __forceinline__ float buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ double buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, double) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
 // value of lane i (wave-uniform i) as a scalar
 __device__ __forceinline__ float lane_value(float v, uint32_t i) {
     return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), i));

@@ -27,9 +27,10 @@ __device__ __forceinline__ void compact_tile(const JoinArgs &a, const uint32_t t
     const uint32_t cap = uniform(a.slotCap[tile]);
     const uint4 tl = a.tiles[tile];
     const uint64_t sbase = uniform64(a.slotOff[tile]);
-    if (used > cap) return;  // overflowed: completed by the fill pass
+    if (used > cap || used > SLOT_POS_LIMIT) return;  // overflowed: completed by the fill pass
     const uint32_t q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t qbits = nq > 1 ? 32u - (uint32_t)__builtin_clz(nq - 1u) : 0u;
     // first batch of entries, loaded beside the rows' offsets
     bool vn = lane < used;
     uint32_t qn = vn ? (uint32_t)a.slotQ[sbase + lane] : 0u;
@@ -50,8 +51,8 @@ __device__ __forceinline__ void compact_tile(const JoinArgs &a, const uint32_t t
             idn = a.slotIds[sbase + e + GDSJ_WAVE + lane];
         }
         uint64_t peers = ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 7; b++) {
+        // the query indices are below nq: only their low qbits bits differ
+        for (uint32_t b = 0; b < qbits; b++) {
             const bool bit = (qi >> b) & 1u;
             const uint64_t mb = ballot(bit);
             peers &= bit ? mb : ~mb;

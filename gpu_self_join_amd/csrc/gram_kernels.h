@@ -147,6 +147,7 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     uint64_t sbase = 0;
     if (PASS == PASS_SLOT) {
         scap = uniform(a.slotCap[tile]);
+        scap = scap < SLOT_POS_LIMIT ? scap : SLOT_POS_LIMIT;  // as the plain kernels' slotted pass
         sbase = uniform64(a.slotOff[tile]);
     }
     uint64_t work = 0;

@@ -26,6 +26,10 @@ enum JoinPass : int { PASS_COUNT = 0, PASS_FILL = 1, PASS_SLOT = 2 };
 
 // Everything the join kernels read besides the coordinates and ids
 // (passed by value as one kernel argument).
+// slot positions of a tile stay below this (their byte offsets fit 32 bits);
+// a tile with more hits than min(slot capacity, limit) has overflowed
+constexpr uint32_t SLOT_POS_LIMIT = 1u << 30;
+
 struct JoinArgs {
     const void *xs;
     uint64_t ld;

@@ -321,6 +321,10 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
     constexpr bool MFILL = HM && PASS == PASS_FILL;  // hits read from the masks
     const uint32_t lane = lane_id();
     const uint64_t ld = a.ld;
+    // the candidates' coordinates by buffer loads (every column at a 32-bit
+    // byte offset from xs) and their ids at 32-bit offsets, where that fits
+    const bool off32 = D > 0 && (uint64_t)(D > 0 ? D : 1) * ld * sizeof(T) <= 0xFFFFFFFFull;
+    const __amdgpu_buffer_rsrc_t xrs = buffer_rsrc(xs);
     const T thr = (T)a.thr;
     const uint4 tl = a.tiles[tile];
     const uint32_t cell = uniform(tl.x), q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
@@ -391,10 +395,16 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
     uint64_t work = 0;
     if (PASS == PASS_FILL && lane < nq) cur_l = a.offsets[qid_l];
     uint32_t scur = 0, scap = 0;
-    uint64_t sbase = 0;
+    uint32_t *sid = nullptr;  // the tile's slot entries (wave-uniform bases)
+    uint8_t *sq = nullptr;
     if (PASS == PASS_SLOT) {
+        // positions stay below SLOT_POS_LIMIT, so their byte offsets fit
+        // 32 bits; a tile with more hits is an overflowed tile
         scap = uniform(a.slotCap[tile]);
-        sbase = uniform64(a.slotOff[tile]);
+        scap = scap < SLOT_POS_LIMIT ? scap : SLOT_POS_LIMIT;
+        const uint64_t sbase = uniform64(a.slotOff[tile]);
+        sid = a.slotIds + sbase;
+        sq = a.slotQ + sbase;
     }
     bool overflow = false;
     // hit masks: word of (query qi, chunk j) = hbase + qi * hstride + j
@@ -491,9 +501,16 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                 if (!MFILL && D > 0 && ch < (int)nch) {
                     // lanes past the end read the NaN padding behind the last
                     // point (position ld - 1), which fails every compare
-                    const uint64_t xp = valid[ch] ? (uint64_t)sp : ld - 1;
+                    if (off32) {
+                        const uint32_t bo = (valid[ch] ? sp : (uint32_t)(ld - 1)) * (uint32_t)sizeof(T);
 #pragma unroll
-                    for (int l = 0; l < (D > 0 ? D : 1); l++) c[ch][l] = xs[(uint64_t)l * ld + xp];
+                        for (int l = 0; l < (D > 0 ? D : 1); l++)
+                            c[ch][l] = buffer_load(xrs, bo, (uint32_t)((uint64_t)l * ld * sizeof(T)), T());
+                    } else {
+                        const uint64_t xp = valid[ch] ? (uint64_t)sp : ld - 1;
+#pragma unroll
+                        for (int l = 0; l < (D > 0 ? D : 1); l++) c[ch][l] = xs[(uint64_t)l * ld + xp];
+                    }
                     if (GVF) {
                         T nn = 0;
 #pragma unroll
@@ -504,7 +521,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                         cn[ch] = nn;
                     }
                 }
-                if (NEED_IDS) cid[ch] = (ch < (int)nch) ? ids[sp] : 0u;
+                if (NEED_IDS) cid[ch] = (ch < (int)nch) ? (off32 ? load_at(ids, sp * 4u) : ids[sp]) : 0u;
                 if (SRT) scell[ch] = (ch < (int)nch) ? a.posCell[sp] : 0u;
             }
             // unswitched on the fill pass's capacity check where it pays

@@ -1,4 +1,4 @@
-// scan.h -- device-wide exclusive prefix sums (three-phase, 2048-item tiles).
+// scan.h -- device-wide exclusive prefix sums (2048-item tiles).
 // Used for CSR offsets, cell heads, neighbor runs, tiles and radix digits.
 #pragma once
 #include "common.h"
@@ -8,6 +8,8 @@ namespace gdsj {
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+// tiles up to which each down-sweep block sums the tile sums before it
+constexpr uint32_t SCAN_DIRECT_TILES = 1024;
 
 template <typename Acc>
 __device__ __forceinline__ Acc wave_inclusive_scan(Acc v) {

@@ -1,5 +1,6 @@
-// scan.hip -- three-phase exclusive scan: tile reduce, scan of tile sums,
-// tile scan plus carry-in.
+// scan.hip -- exclusive scan: tile reduce, then tile scan plus carry-in,
+// the carry summed by each block itself (up to SCAN_DIRECT_TILES tiles) or
+// taken from a scan of the tile sums.
 #include "scan.h"
 
 namespace gdsj {
@@ -44,6 +45,44 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_partials(Acc *__restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) partial[nb] = carry;
+}
+
+// down-sweep whose block sums the tile sums before its tile itself (no
+// separate scan of the tile sums: one launch less, for up to
+// SCAN_DIRECT_TILES tiles)
+template <typename In, typename Out, typename Acc>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down_direct(const In *__restrict__ in, Out *__restrict__ out,
+                                                                  uint64_t n, const Acc *__restrict__ partial,
+                                                                  int write_total) {
+    Acc c = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += SCAN_BLOCK) c += partial[i];
+    Acc carry;
+    (void)block_exclusive_scan<Acc>(c, &carry);
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    Acc v[SCAN_ITEMS];
+    Acc loc = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        uint64_t idx = base + i;
+        v[i] = idx < n ? (Acc)in[idx] : 0;
+        loc += v[i];
+    }
+    Acc tot;
+    Acc ex = block_exclusive_scan<Acc>(loc, &tot) + carry;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        uint64_t idx = base + i;
+        if (idx < n) out[idx] = (Out)ex;
+        ex += v[i];
+    }
+    if (write_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = (Out)(carry + tot);
+}
+
+// writes the total of an empty scan (a kernel, not a memset, so that the
+// scan can be captured in a graph)
+template <typename Out>
+__global__ void k_scan_empty(Out *__restrict__ out) {
+    if (threadIdx.x == 0) out[0] = 0;
 }
 
 template <typename In, typename Out, typename Acc>
@@ -101,12 +140,17 @@ static int scan_impl(const uint32_t *in, Out *out, uint64_t n, bool write_total,
     uint32_t nb = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
     Acc *partial = (Acc *)tmp;
     if (n == 0) {
-        if (write_total) GDSJ_HIP(hipMemsetAsync(out, 0, sizeof(Out), s));
+        if (write_total) k_scan_empty<Out><<<1, 64, 0, s>>>(out);
+        GDSJ_HIP(hipGetLastError());
         return GDSJ_OK;
     }
     k_scan_reduce<uint32_t, Acc><<<nb, SCAN_BLOCK, 0, s>>>(in, n, partial);
-    k_scan_partials<Acc><<<1, SCAN_BLOCK, 0, s>>>(partial, nb);
-    k_scan_down<uint32_t, Out, Acc><<<nb, SCAN_BLOCK, 0, s>>>(in, out, n, partial, nb, write_total ? 1 : 0);
+    if (nb <= SCAN_DIRECT_TILES) {
+        k_scan_down_direct<uint32_t, Out, Acc><<<nb, SCAN_BLOCK, 0, s>>>(in, out, n, partial, write_total ? 1 : 0);
+    } else {
+        k_scan_partials<Acc><<<1, SCAN_BLOCK, 0, s>>>(partial, nb);
+        k_scan_down<uint32_t, Out, Acc><<<nb, SCAN_BLOCK, 0, s>>>(in, out, n, partial, nb, write_total ? 1 : 0);
+    }
     GDSJ_HIP(hipGetLastError());
     return GDSJ_OK;
 }
