@@ -313,7 +313,7 @@ def host_path(x, eps, kw, steps=2, warmup=1):
     return {"value": len(table) / wall, "unit": "pairs/s", "ms_per_join": round(1e3 * wall, 3), "steps": steps,
             "ms_join_call": round(1e3 * (t2 - t1), 3), "ms_table_copy": round(1e3 * (t3 - t2), 3),
             "table_bytes": int(tab.nbytes),
-            "path": "gdsjoin(): host array in, counts and table out over PCIe (pinned staging, 8 copy threads)"}
+            "path": "gdsjoin(): host array in, counts and table out over PCIe (a ring of pinned buffers: one DMA stream in chunk order, up to 8 host copy threads)"}
 
 
 def estimator_holdout(name, x, eps, base):

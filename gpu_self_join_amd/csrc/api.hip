@@ -42,13 +42,14 @@ void set_error(const char *fmt, ...) {
 
 using namespace gdsj;
 
-// Host-buffer copies (gdsjoin's path) run through pinned staging:
-// STAGE_THREADS host threads each move every STAGE_THREADS-th chunk of
-// STAGE_CHUNK bytes through two pinned buffers on a stream of their own, so
-// that DMA of one chunk overlaps the host memcpy of the previous one and the
-// threads' transfers overlap each other.
+// Host-buffer copies (gdsjoin's path) run through a ring of STAGE_RING
+// pinned buffers of STAGE_CHUNK bytes: one stream moves the chunks in order
+// (the link stays busy) while up to STAGE_THREADS host threads copy chunks
+// out of (or into) the buffers, chunk c through buffer c % STAGE_RING by
+// thread c % threads, and hand each buffer back for chunk c + STAGE_RING.
 constexpr int STAGE_THREADS = 8;
-constexpr size_t STAGE_CHUNK = 8u << 20;
+constexpr int STAGE_RING = 16;
+constexpr size_t STAGE_CHUNK = 4u << 20;
 
 struct gdsj_handle {
     int device = 0;
@@ -90,9 +91,9 @@ struct gdsj_handle {
     hipEvent_t ev[9] = {};  // 8: start of the row sort
     // host-buffer copies: per copy thread a stream, two pinned chunk buffers
     // and their events (staged_copy)
-    hipStream_t cstream[STAGE_THREADS] = {};
-    hipEvent_t cev[STAGE_THREADS][2] = {};
-    HostBuf pin[STAGE_THREADS][2];
+    hipStream_t cstream = nullptr;
+    hipEvent_t cev[STAGE_RING] = {};
+    HostBuf pin[STAGE_RING];
 };
 
 static int staged_copy(gdsj_handle *h, void *dst, const void *src, size_t bytes, bool d2h);
@@ -193,14 +194,12 @@ int gdsj_destroy(gdsj_handle *h) {
     for (DevBuf *x : b) x->release();
     for (int i = 0; i < 9; i++)
         if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
-    for (int t = 0; t < STAGE_THREADS; t++) {
-        if (h->cstream[t]) (void)hipStreamSynchronize(h->cstream[t]);
-        for (int b = 0; b < 2; b++) {
-            if (h->cev[t][b]) (void)hipEventDestroy(h->cev[t][b]);
-            h->pin[t][b].release();
-        }
-        if (h->cstream[t]) (void)hipStreamDestroy(h->cstream[t]);
+    if (h->cstream) (void)hipStreamSynchronize(h->cstream);
+    for (int k = 0; k < STAGE_RING; k++) {
+        if (h->cev[k]) (void)hipEventDestroy(h->cev[k]);
+        h->pin[k].release();
     }
+    if (h->cstream) (void)hipStreamDestroy(h->cstream);
     if (h->own) (void)hipStreamDestroy(h->own);
     delete h;
     return GDSJ_OK;
@@ -833,71 +832,91 @@ static int staged_copy(gdsj_handle *h, void *dst, const void *src, size_t bytes,
         GDSJ_HIP(hipMemcpy(dst, src, bytes, d2h ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice));
         return GDSJ_OK;
     }
-    const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
-    const int nt = (int)(nch < (size_t)STAGE_THREADS ? nch : (size_t)STAGE_THREADS);
-    for (int t = 0; t < nt; t++) {
-        if (!h->cstream[t]) GDSJ_HIP(hipStreamCreateWithFlags(&h->cstream[t], hipStreamNonBlocking));
-        for (int b = 0; b < 2; b++) {
-            if (!h->cev[t][b]) GDSJ_HIP(hipEventCreateWithFlags(&h->cev[t][b], hipEventDisableTiming));
-            GDSJ_TRY(h->pin[t][b].ensure(STAGE_CHUNK));
-        }
+    const int64_t nch = (int64_t)((bytes + STAGE_CHUNK - 1) / STAGE_CHUNK);
+    if (!h->cstream) GDSJ_HIP(hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking));
+    for (int k = 0; k < STAGE_RING; k++) {
+        if (!h->cev[k]) GDSJ_HIP(hipEventCreateWithFlags(&h->cev[k], hipEventDisableTiming));
+        GDSJ_TRY(h->pin[k].ensure(STAGE_CHUNK));
+    }
+    const unsigned hc = std::thread::hardware_concurrency();
+    int nt = hc >= 2 && hc - 1 < (unsigned)STAGE_THREADS ? (int)hc - 1 : STAGE_THREADS;
+    if ((int64_t)nt > nch) nt = (int)nch;
+    // per buffer: the last chunk the stream took (issued) and the last chunk
+    // the host side finished with it (done); -1: none
+    std::atomic<int64_t> issued[STAGE_RING], done[STAGE_RING];
+    for (int k = 0; k < STAGE_RING; k++) {
+        issued[k].store(-1);
+        done[k].store(-1);
     }
     std::atomic<int> failed{0};
-    auto work = [&](int t) {
+    hipStream_t cs = h->cstream;
+    auto chunk_len = [&](int64_t c) {
+        const size_t off = (size_t)c * STAGE_CHUNK;
+        return bytes - off < STAGE_CHUNK ? bytes - off : STAGE_CHUNK;
+    };
+    auto wait_for = [&](std::atomic<int64_t> &v, int64_t want) {
+        while (v.load(std::memory_order_acquire) < want) {
+            if (failed.load(std::memory_order_relaxed)) return false;
+            std::this_thread::yield();
+        }
+        return true;
+    };
+    auto host_side = [&](int t) {
         if (hipSetDevice(h->device) != hipSuccess) {
             failed = 1;
             return;
         }
-        hipStream_t cs = h->cstream[t];
-        size_t prev_off = 0, prev_len = 0;
-        int i = 0;
-        for (size_t c = (size_t)t; c < nch; c += (size_t)nt, i++) {
-            const int b = i & 1;
-            const size_t off = c * STAGE_CHUNK;
-            const size_t len = bytes - off < STAGE_CHUNK ? bytes - off : STAGE_CHUNK;
-            void *pb = h->pin[t][b].p;
+        for (int64_t c = t; c < nch; c += nt) {
+            const int k = (int)(c % STAGE_RING);
+            const size_t off = (size_t)c * STAGE_CHUNK, len = chunk_len(c);
             if (d2h) {
-                if (hipMemcpyAsync(pb, (const char *)src + off, len, hipMemcpyDeviceToHost, cs) != hipSuccess ||
-                    hipEventRecord(h->cev[t][b], cs) != hipSuccess) {
+                // the stream has taken chunk c into buffer k: wait for its copy
+                if (!wait_for(issued[k], c)) return;
+                if (hipEventSynchronize(h->cev[k]) != hipSuccess) {
                     failed = 1;
                     return;
                 }
-                if (i > 0) {
-                    if (hipEventSynchronize(h->cev[t][b ^ 1]) != hipSuccess) {
+                memcpy((char *)dst + off, h->pin[k].p, len);
+            } else {
+                // buffer k is free once the stream has sent chunk c - STAGE_RING
+                if (c >= STAGE_RING) {
+                    if (!wait_for(issued[k], c - STAGE_RING)) return;
+                    if (hipEventSynchronize(h->cev[k]) != hipSuccess) {
                         failed = 1;
                         return;
                     }
-                    memcpy((char *)dst + prev_off, h->pin[t][b ^ 1].p, prev_len);
                 }
-                prev_off = off;
-                prev_len = len;
-            } else {
-                // buffer b was last handed to the DMA two chunks ago
-                if (i >= 2 && hipEventSynchronize(h->cev[t][b]) != hipSuccess) {
-                    failed = 1;
-                    return;
-                }
-                memcpy(pb, (const char *)src + off, len);
-                if (hipMemcpyAsync((char *)dst + off, pb, len, hipMemcpyHostToDevice, cs) != hipSuccess ||
-                    hipEventRecord(h->cev[t][b], cs) != hipSuccess) {
-                    failed = 1;
-                    return;
-                }
+                memcpy(h->pin[k].p, (const char *)src + off, len);
             }
+            done[k].store(c, std::memory_order_release);
         }
-        if (d2h && i > 0) {
-            if (hipEventSynchronize(h->cev[t][(i - 1) & 1]) != hipSuccess) {
-                failed = 1;
-                return;
-            }
-            memcpy((char *)dst + prev_off, h->pin[t][(i - 1) & 1].p, prev_len);
-        }
-        if (hipStreamSynchronize(cs) != hipSuccess) failed = 1;
     };
     std::vector<std::thread> th;
-    for (int t = 1; t < nt; t++) th.emplace_back(work, t);
-    work(0);
+    for (int t = 0; t < nt; t++) th.emplace_back(host_side, t);
+    // this thread drives the stream, chunks in order
+    for (int64_t c = 0; c < nch && !failed.load(); c++) {
+        const int k = (int)(c % STAGE_RING);
+        const size_t off = (size_t)c * STAGE_CHUNK, len = chunk_len(c);
+        bool ok;
+        if (d2h) {
+            // buffer k is free once the host side has copied chunk c - STAGE_RING out
+            ok = c < STAGE_RING || wait_for(done[k], c - STAGE_RING);
+            ok = ok && hipMemcpyAsync(h->pin[k].p, (const char *)src + off, len, hipMemcpyDeviceToHost, cs) ==
+                           hipSuccess;
+        } else {
+            // chunk c is in buffer k once the host side has filled it
+            ok = wait_for(done[k], c);
+            ok = ok && hipMemcpyAsync((char *)dst + off, h->pin[k].p, len, hipMemcpyHostToDevice, cs) == hipSuccess;
+        }
+        ok = ok && hipEventRecord(h->cev[k], cs) == hipSuccess;
+        if (!ok) {
+            failed = 1;
+            break;
+        }
+        issued[k].store(c, std::memory_order_release);
+    }
     for (auto &x : th) x.join();
+    if (hipStreamSynchronize(cs) != hipSuccess) failed = 1;
     GDSJ_HIP(hipSetDevice(h->device));
     if (failed) {
         set_error("staged %s copy of %zu bytes failed", d2h ? "device-to-host" : "host-to-device", bytes);

@@ -57,6 +57,25 @@ def test_readme_through_gdsjoin_dropin(gpu):
     assert_csr_equal(counts, table, ref["counts"], ref["ids"])
 
 
+def test_host_path_staging_ring_equals_device_join(sj):
+    """The drop-in's host path moves the points in and the table out through
+    a ring of pinned buffers (one stream, host copy threads): 72 MB of points
+    and a table of about 200 MB go round it more than once.  The counts and
+    rows must equal the device-resident join of the same points."""
+    import torch
+    rng = np.random.default_rng(21)
+    x = (rng.random((4_500_000, 4)) * np.array([60.0, 60.0, 60.0, 60.0])).astype(np.float32)
+    eps = 1.6
+    kw = dict(k=4, ilp=0, shortcircuit=0, reorder=0)
+    counts, table = gdsjoin(x.ravel(), eps, 4, "float", verbose=False, params=_lib.default_params(**kw))
+    print("host path pairs", len(table), flush=True)
+    assert x.nbytes > 16 * (4 << 20) and table.nbytes > 16 * (4 << 20)
+    res = sj.join(torch.from_numpy(x).cuda(), eps, **kw)
+    dc = res.counts.cpu().numpy().view(np.uint32)
+    di = res.ids[: res.total].cpu().numpy().view(np.uint32)
+    assert_csr_equal(counts, table, dc, di)
+
+
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 def test_ilp_fold_readings_on_straddling_pairs(sj, dt):
     """Data with pairs on which the two readings of the ILP fold disagree:
