@@ -68,6 +68,13 @@ struct gdsj_handle {
     bool stamp = false;  // unicomp in force for this join
     DevBuf fcnt, rcur;   // unicomp: own-side counts, reverse-entry cursors
     DevBuf maskW, hmCell, hmask;  // hit masks of the exact passes
+    // matrix-core join (gdsj_params.mfma): tile lists of the last grid
+    DevBuf mjList, mjFlags;
+    hipStream_t mjs = nullptr;  // the matrix-core tiles' stream (default priority)
+    hipEvent_t mjFork = nullptr, mjJoin = nullptr;
+    bool use_mj = false;
+    uint32_t mj_nbig = 0;
+    float mj_hi = 0.0f, mj_lo = 0.0f;
     DevBuf bfXs;                  // all-pairs join: SoA copy of the points
     bool use_hm = false;
     bool slotted = false;
@@ -184,6 +191,8 @@ int gdsj_destroy(gdsj_handle *h) {
     h->bfXs.release();
     h->hmCell.release();
     h->hmask.release();
+    h->mjList.release();
+    h->mjFlags.release();
     DevBuf *b[] = {&h->counts,  &h->offsets,  &h->ids,     &h->data,    &h->scanTmp, &h->status,
                    &h->work,    &h->slotCap,  &h->slotOff, &h->slotUsed, &h->slotOvf, &h->slotIds,
                    &h->slotQ,   &h->scalars,  &h->estPred, &h->estScratch, &h->subTiles,
@@ -200,6 +209,12 @@ int gdsj_destroy(gdsj_handle *h) {
         h->pin[k].release();
     }
     if (h->cstream) (void)hipStreamDestroy(h->cstream);
+    if (h->mjs) {
+        (void)hipStreamSynchronize(h->mjs);
+        (void)hipStreamDestroy(h->mjs);
+    }
+    if (h->mjFork) (void)hipEventDestroy(h->mjFork);
+    if (h->mjJoin) (void)hipEventDestroy(h->mjJoin);
     if (h->own) (void)hipStreamDestroy(h->own);
     delete h;
     return GDSJ_OK;
@@ -257,6 +272,10 @@ static JoinArgs make_args(gdsj_handle *h) {
     JoinArgs a{};
     a.xs = h->G.xs.p;
     a.ld = h->G.ld;
+    if (h->G.xaStride) {
+        a.xa = h->G.xa.p;
+        a.xaStride = h->G.xaStride;
+    }
     a.ids = h->G.ids.as<uint32_t>();
     a.tiles = h->G.tiles.as<uint4>();
     a.ntiles = h->G.ntiles;
@@ -286,6 +305,8 @@ static JoinArgs make_args(gdsj_handle *h) {
     // slotted pass it issues 30 % fewer VALU instructions but runs 15 %
     // slower, its per-chunk band test and branches serialising the waves)
     if (h->p.vfilter == 1 && !h->p.sort && gram_valu_band(h->G, h->thr, &a.gvLo, &a.gvMid, &a.gvHalf)) a.gv = 1;
+    a.mjHi = h->mj_hi;
+    a.mjLo = h->mj_lo;
     if (h->p.sort) {
         a.sortMode = h->G.k == h->G.d ? 1u : 2u;
         a.sortDim = h->G.k == h->G.d ? 0u : h->G.k;
@@ -302,6 +323,20 @@ static JoinArgs make_args(gdsj_handle *h) {
         a.cellCand = h->G.cellCand.as<uint32_t>();
     }
     return a;
+}
+
+// The passes over the grid's own tiles split them between the matrix-core
+// join and the plain kernels (sub-tiles and sampled tiles keep the plain
+// kernels).
+static void set_mj_lists(gdsj_handle *h, JoinArgs &a) {
+    if (!h->use_mj) return;
+    a.mjList = h->mjList.as<uint32_t>();
+    a.mjN = h->mj_nbig;
+    a.vjList = a.mjList + h->mj_nbig;
+    a.vjN = h->G.ntiles - h->mj_nbig;
+    a.mjStream = h->mjs;
+    a.mjFork = h->mjFork;
+    a.mjJoin = h->mjJoin;
 }
 
 // Writes the table of the last join into d_ids: the exact fill pass, or,
@@ -385,7 +420,9 @@ static int fill_enqueue(gdsj_handle *h, uint32_t *d_ids, uint64_t cap) {
             }
         }
     } else {
-        GDSJ_TRY(run_pass(h, a, PASS_FILL));
+        JoinArgs f = a;
+        set_mj_lists(h, f);
+        GDSJ_TRY(run_pass(h, f, PASS_FILL));
         GDSJ_HIP(hipEventRecord(h->ev[7], s));
     }
     GDSJ_HIP(hipEventRecord(h->ev[5], s));
@@ -462,7 +499,9 @@ static int slotted_pass(gdsj_handle *h, JoinArgs &a, bool *used) {
     GDSJ_HIP(hipMemsetAsync(h->scalars.p, 0, 4, s));
     GDSJ_HIP(hipEventRecord(h->ev[2], s));
     set_slot_args(h, a);
-    GDSJ_TRY(run_pass(h, a, PASS_SLOT));
+    JoinArgs m = a;
+    set_mj_lists(h, m);
+    GDSJ_TRY(run_pass(h, m, PASS_SLOT));
     *used = true;
     return GDSJ_OK;
 }
@@ -530,6 +569,8 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
         tq = 64;  // the plain kernels keep one query per lane
     }
     h->G.sortCells = h->p.sort ? 1 : 0;
+    // the row-major copy for the plain kernels' candidate loads (up to 8 dims)
+    h->G.aosWant = !h->use_gram && dims <= 8 ? 1 : 0;
     GDSJ_TRY(grid_build(h->G, d_data, n, dims, dtype, epsilon, h->p.k, h->p.reorder, tq, h->p.queryreorder, s));
     if (h->use_gram) {
         GDSJ_TRY(gram_prepare(h->gram, h->G, gram_lowp, s));
@@ -574,6 +615,26 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
             return GDSJ_EUNSUPPORTED;
         }
     }
+    // the matrix-core join: every dim indexed, 3 to 7 dims, plain kernels
+    // over precomputed runs, tiles of at most 64 queries
+    h->use_mj = false;
+    h->mj_nbig = 0;
+    if (h->p.mfma != 0 && h->p.mfma != 2 && !h->use_gram && !h->stamp && !h->p.sort && h->G.haveRuns &&
+        h->p.vfilter != 1 && tq <= 64 && h->G.ntiles > 0 && mjoin_band(h->G, h->thr, &h->mj_hi, &h->mj_lo)) {
+        const uint32_t nt = h->G.ntiles;
+        const uint32_t minOcc = h->p.mfma == 1 ? 1u : h->p.mfma;
+        GDSJ_TRY(h->mjList.ensure((size_t)nt * 4 + 256));
+        GDSJ_TRY(h->mjFlags.ensure(((size_t)nt + 1) * 8 + 256));
+        GDSJ_TRY(h->scanTmp.ensure(scan_tmp_bytes((uint64_t)nt + 1) + scan_tmp_bytes((uint64_t)n + 1) + 256));
+        GDSJ_TRY(mjoin_partition(h->G, minOcc, h->mjList.as<uint32_t>(), h->mjFlags.as<uint32_t>(), h->scanTmp.p,
+                                 &h->mj_nbig, s));
+        h->use_mj = h->mj_nbig > 0;
+        if (h->use_mj && !h->mjs) {
+            GDSJ_HIP(hipStreamCreateWithFlags(&h->mjs, hipStreamNonBlocking));
+            GDSJ_HIP(hipEventCreateWithFlags(&h->mjFork, hipEventDisableTiming));
+            GDSJ_HIP(hipEventCreateWithFlags(&h->mjJoin, hipEventDisableTiming));
+        }
+    }
     h->have_grid = true;
     return join_passes(h, n, dims, d_counts, d_offsets, d_ids, ids_cap, total);
 }
@@ -606,7 +667,9 @@ static int join_passes(gdsj_handle *h, uint32_t n, uint32_t dims, uint32_t *d_co
         GDSJ_HIP(hipEventRecord(h->ev[1], s));
         GDSJ_HIP(hipEventRecord(h->ev[2], s));
         if (h->stamp && n) GDSJ_HIP(hipMemsetAsync(d_counts, 0, (size_t)n * 4, s));  // accumulated by atomics
-        GDSJ_TRY(run_pass(h, a, PASS_COUNT));
+        JoinArgs m = a;
+        set_mj_lists(h, m);
+        GDSJ_TRY(run_pass(h, m, PASS_COUNT));
     }
     h->slotted = slotted;
     GDSJ_HIP(hipEventRecord(h->ev[3], s));
@@ -637,6 +700,7 @@ static int join_passes(gdsj_handle *h, uint32_t n, uint32_t dims, uint32_t *d_co
     h->st.n_tiles = h->G.ntiles;
     h->st.n_runs = h->G.haveRuns ? 0 : (uint64_t)h->G.ntiles * a.ncombo;
     h->st.est_overflow_tiles = h->novf;
+    h->st.mfma_tiles = h->use_mj ? h->mj_nbig : 0;
     h->st.filter_rechecks = gst[0];
     h->st.filter_hits = gst[1];
     memcpy(&h->st.filter_max_err, &gst[2], 8);

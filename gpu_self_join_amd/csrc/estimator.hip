@@ -501,6 +501,7 @@ int estimator_capacity_sampling(const Grid &G, const void *joinArgs, int mode, u
         JoinArgs a = *(const JoinArgs *)joinArgs;
         a.tiles = stiles;
         a.tileList = nullptr;
+        a.mjList = nullptr;
         a.ntiles = ns;
         a.counts = countsScratch;
         a.tileWork = nullptr;

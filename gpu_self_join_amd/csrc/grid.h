@@ -64,6 +64,14 @@ struct Grid {
     // REORDER); posCell maps a sorted position to its cell
     int sortCells = 0;
     DevBuf posCell;
+    // row-major copy of xs (set aosWant before grid_build; up to 8 dims): point
+    // p's coordinates at xa[p * xaStride ..], xaStride = dims rounded up to
+    // 2, 4 or 8, zero-padded, NaN in the padding rows n .. ld-1.  A
+    // candidate's coordinates then share one or two cache lines instead of
+    // one per dim (runs of adjacent cells are short in 6 dims).
+    int aosWant = 0;
+    uint32_t xaStride = 0;
+    DevBuf xa;
     DevBuf keyA, keyB, valB, tmp, aux, flag, tileKey, tileVal, cellAux, scalars;
     HostBuf hostSample;  // REORDER's sample rows, pinned (the variance is summed on the host)
     void release();

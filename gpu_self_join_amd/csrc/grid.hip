@@ -218,15 +218,20 @@ __global__ __launch_bounds__(256) void k_soa_tile(const T *__restrict__ data, ui
                                                   const uint32_t *__restrict__ perm,
                                                   const uint32_t *__restrict__ order, T *__restrict__ xs,
                                                   uint64_t ld, uint32_t *__restrict__ ids,
-                                                  uint32_t *__restrict__ nonfinite) {
+                                                  uint32_t *__restrict__ nonfinite, T *__restrict__ xa,
+                                                  uint32_t as) {
     extern __shared__ __align__(16) unsigned char soa_lds[];
     T *tile = reinterpret_cast<T *>(soa_lds);
     __shared__ uint32_t s_id[TP];
     __shared__ uint32_t s_perm[MAXD];
     __shared__ uint32_t s_bad;
-    if (blockIdx.x == 0)  // NaN padding at positions n .. ld-1
+    if (blockIdx.x == 0) {  // NaN padding at positions n .. ld-1
         for (uint64_t t = threadIdx.x; t < (ld - n) * d; t += blockDim.x)
             xs[(t / (ld - n)) * ld + n + t % (ld - n)] = (T)__builtin_nan("");
+        if (xa)
+            for (uint64_t t = threadIdx.x; t < (ld - n) * as; t += blockDim.x)
+                xa[(uint64_t)n * as + t] = t % as < d ? (T)__builtin_nan("") : (T)0;
+    }
     const uint32_t s0 = blockIdx.x * TP;
     const uint32_t np = n - s0 < TP ? n - s0 : TP;
     if (threadIdx.x == 0) s_bad = 0;
@@ -250,6 +255,11 @@ __global__ __launch_bounds__(256) void k_soa_tile(const T *__restrict__ data, ui
         const uint32_t l = t / TP, p = t % TP;
         if (p < np) xs[(uint64_t)l * ld + s0 + p] = tile[l * (TP + 1) + p];
     }
+    if (xa)
+        for (uint32_t t = threadIdx.x; t < np * as; t += blockDim.x) {
+            const uint32_t p = t / as, l = t - p * as;
+            xa[(uint64_t)(s0 + p) * as + l] = l < d ? tile[l * (TP + 1) + p] : (T)0;
+        }
     if (threadIdx.x == 0 && s_bad) atomicOr(nonfinite, 1u);
 }
 
@@ -339,7 +349,7 @@ __global__ void k_tiles_permute(const uint4 *__restrict__ in, const uint32_t *__
 void Grid::release() {
     DevBuf *b[] = {&xs, &ids, &cellKey, &cellStart, &cellCoord, &hkeys, &hvals, &tiles, &keyA, &keyB,
                    &valB, &tmp, &aux, &flag, &tileKey, &tileVal, &cellAux, &scalars, &runs, &nruns, &cellCand, &cellHist,
-                   &comboTab, &posCell};
+                   &comboTab, &posCell, &xa};
     for (DevBuf *p : b) p->release();
     hostSample.release();
 }
@@ -496,14 +506,20 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     uint32_t *nonfinite = G.scalars.as<uint32_t>();
     uint32_t *idsOut = svals == vA ? vB : vA;  // the value buffer the sort did not end in
     GDSJ_HIP(hipMemsetAsync(nonfinite, 0, 4, s));
+    G.xaStride = G.aosWant && d <= 8 ? (d <= 2 ? 2u : (d <= 4 ? 4u : 8u)) : 0u;
+    T *xa = nullptr;
+    if (G.xaStride) {
+        GDSJ_TRY(G.xa.ensure((size_t)G.ld * G.xaStride * sizeof(T) + 256));
+        xa = G.xa.as<T>();
+    }
     if ((size_t)d * 65 * sizeof(T) <= 48 * 1024)
         k_soa_tile<T, 64><<<ceil_div(n, 64), 256, (size_t)d * 65 * sizeof(T), s>>>(data, n, d, dperm, svals,
                                                                                      G.xs.as<T>(), G.ld, idsOut,
-                                                                                     nonfinite);
+                                                                                     nonfinite, xa, G.xaStride);
     else if ((size_t)d * 33 * sizeof(T) <= 48 * 1024)
         k_soa_tile<T, 32><<<ceil_div(n, 32), 256, (size_t)d * 33 * sizeof(T), s>>>(data, n, d, dperm, svals,
                                                                                      G.xs.as<T>(), G.ld, idsOut,
-                                                                                     nonfinite);
+                                                                                     nonfinite, xa, G.xaStride);
     else
         k_soa<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, dperm, svals, G.xs.as<T>(), G.ld, idsOut, nonfinite);
     GDSJ_HIP(hipGetLastError());

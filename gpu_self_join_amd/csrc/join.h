@@ -33,6 +33,10 @@ constexpr uint32_t SLOT_POS_LIMIT = 1u << 30;
 struct JoinArgs {
     const void *xs;
     uint64_t ld;
+    // nullable: the row-major copy of xs (Grid::xa; up to 8 dims), from which
+    // the kernels load candidate coordinates; xaStride elements per point
+    const void *xa;
+    uint32_t xaStride;
     const uint32_t *ids;
     const uint4 *tiles;
     const uint32_t *tileList;  // nullable: process tiles[tileList[t]] for t < ntiles
@@ -108,6 +112,23 @@ struct JoinArgs {
     uint32_t winSplit;          // 0: tiles are not split
     uint32_t *splitCnt;
     const uint64_t *splitOff;
+    // matrix-core join of low-dimensional data (every dim indexed, 3 to 7
+    // dims; k_mjoin): the tiles mjList[0 .. mjN) run on it, the tiles
+    // vjList[0 .. vjN) on the plain kernels (mjN + vjN = ntiles, both lists in
+    // tile order).  A pair whose Gram-form estimate F exceeds mjHi is not a
+    // neighbor, one with F <= mjLo is (mjoin_band); mjList = nullptr: the
+    // plain kernels for every tile.
+    const uint32_t *mjList;
+    uint32_t mjN;
+    const uint32_t *vjList;
+    uint32_t vjN;
+    float mjHi, mjLo;
+    // (host side) the matrix-core tiles run on the stream mjStream (a
+    // hipStream_t), forked from and joined back into the pass's stream by
+    // the events mjFork and mjJoin, so that the plain kernels' many small
+    // tiles fill the SIMDs the few heavy matrix-core tiles leave idle;
+    // nullptr: both on the pass's stream, one after the other
+    void *mjStream, *mjFork, *mjJoin;
 };
 
 constexpr uint32_t SPLIT_ROWS = 128;  // row stride of splitCnt / splitOff
@@ -122,6 +143,19 @@ __host__ __device__ __forceinline__ uint32_t hmask_stride(uint32_t cand, uint32_
 // not, and |g - mid| <= half holds for every g in (lo, hi] even after the
 // rounding of g - mid.  False when the filter does not apply.
 bool gram_valu_band(const Grid &G, double thr, float *lo, float *mid, float *half);
+
+// Thresholds of the matrix-core join (k_mjoin) of G (every dim indexed, 3 to
+// 7 dims): pairs with F > hi are not neighbors, pairs with F <= lo are.
+// False when it does not apply (other dims, cells too fine for float
+// coordinates, a band wider than 5 % of thr).
+bool mjoin_band(const Grid &G, double thr, float *hi, float *lo);
+
+// list[0 .. *nbig) = the tiles whose cell holds at least minOcc points,
+// list[*nbig .. ntiles) = the others, both in tile order (tmp: scan scratch
+// of scan_tmp_bytes(ntiles + 1) bytes, flags: 2 (ntiles + 1) words).
+// Waits for the stream (the count comes back to the host).
+int mjoin_partition(const Grid &G, uint32_t minOcc, uint32_t *list, uint32_t *flags, void *tmp, uint32_t *nbig,
+                    hipStream_t s);
 
 // Launches one pass (JoinPass) of the join over a.ntiles tiles on stream s.
 int join_pass(const Grid &G, const JoinArgs &a, int mode, int pass, hipStream_t s);

@@ -2,6 +2,7 @@
 #include <math.h>
 
 #include "join.h"
+#include "scan.h"
 
 namespace gdsj {
 
@@ -64,6 +65,87 @@ bool gram_valu_band(const Grid &G, double thr, float *lo, float *mid, float *hal
     *mid = m;
     *half = hf;
     return true;
+}
+
+// The matrix-core join's band.  A query q and a candidate c of one tile are
+// taken relative to the tile's first query o (every dim indexed, so
+// |q - o| <= rq = eps and |c - o| <= rc = 2 eps up to the rounding of the
+// cell coordinates, slack): a = fl32(fl(q - o)), b = fl32(fl(c - o)), each
+// within 2u of the exact difference (u = 2^-24 >= the DTYPE's roundoff)
+// plus 2^-126 where the matrix core flushes a subnormal.  The kernel forms
+// na = fl |a|^2, nb = fl |b|^2 and F, the MFMA's sum of the K = D + 2
+// products (-2 a_j) b_j, 1 * nb and na * 1.  With X the exact squared
+// distance and acc the reference's sum (|acc - X| <= g X, g = gamma(D+3),
+// kernel.cu:307-357):
+//   |(a - b)^2 - X|                    <= E1 = 2 em D (rq + rc) + D em^2,  em = 2u (rq + rc) + 2^-125
+//   |F - (na + nb - 2 a.b)|            <= gamma(K+1) 2.01 D (Aq^2 + Ac^2) + K 2^-126
+//   |na - |a|^2| + |nb - |b|^2|        <= gamma(D) D (Aq^2 + Ac^2)
+// so |F - X| <= E (their sum).  acc <= thr implies X <= thr / (1 - g) and
+// F <= thr / (1 - g) + E <= hi; F <= lo implies X <= lo + E <=
+// thr / (1 + g) and acc <= thr.  Both margins are taken twice.
+bool mjoin_band(const Grid &G, double thr, float *hi, float *lo) {
+    if (G.k != G.d || G.d < 3 || G.d > 7) return false;
+    const double u = 0x1p-24, eps = G.eps, D = (double)G.d;
+    const double K = 2.0 * (double)((G.d + 3) / 2);
+    if (!(eps > 0x1p-40) || !(eps < 0x1p40)) return false;  // squares stay normal floats
+    double xmax = 0.0;
+    for (uint32_t j = 0; j < G.k; j++) {
+        xmax = fmax(xmax, fabs(G.minArr[j]));
+        xmax = fmax(xmax, fabs(G.minArr[j] + (double)G.ga.nCells[j] * eps));
+    }
+    const double slack = 16.0 * u * (xmax + 2.0 * eps);
+    if (!(slack < 0.01 * eps)) return false;  // cells too fine for float coordinates
+    const double rq = eps + 2.0 * slack, rc = 2.0 * eps + 2.0 * slack;
+    const double aq = rq * (1.0 + 2.0 * u) + 0x1p-126, ac = rc * (1.0 + 2.0 * u) + 0x1p-126;
+    auto gamma = [&](double m) { return m * u / (1.0 - m * u); };
+    const double em = 2.0 * u * (rq + rc) + 0x1p-125;
+    const double e1 = 2.0 * em * D * (rq + rc) + D * em * em;
+    const double e2 = gamma(K + 1.0) * 2.01 * D * (aq * aq + ac * ac) + K * 0x1p-126;
+    const double e3 = gamma(D) * D * (aq * aq + ac * ac);
+    const double E = e1 + e2 + e3;
+    const double g = gamma(D + 3.0);
+    const double th = thr / (1.0 - g) + 2.0 * E;
+    const double tl = thr / (1.0 + g) - 2.0 * E;
+    if (!(th - tl < 0.05 * thr) || !(tl > 0.0)) return false;
+    float hf = (float)th;
+    if ((double)hf < th) hf = nextafterf(hf, INFINITY);
+    float lf = (float)tl;
+    if ((double)lf > tl) lf = nextafterf(lf, -INFINITY);
+    *hi = hf;
+    *lo = lf;
+    return true;
+}
+
+__global__ void k_mj_flags(const uint4 *__restrict__ tiles, uint32_t nt, const uint32_t *__restrict__ cellStart,
+                           uint32_t minOcc, uint32_t *__restrict__ flags) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const uint32_t c = tiles[t].x;
+    flags[t] = cellStart[c + 1] - cellStart[c] >= minOcc ? 1u : 0u;
+}
+
+__global__ void k_mj_scatter(const uint32_t *__restrict__ flags, const uint32_t *__restrict__ pre, uint32_t nt,
+                             uint32_t *__restrict__ list) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const uint32_t nbig = pre[nt];
+    list[flags[t] ? pre[t] : nbig + (t - pre[t])] = t;
+}
+
+int mjoin_partition(const Grid &G, uint32_t minOcc, uint32_t *list, uint32_t *flags, void *tmp, uint32_t *nbig,
+                    hipStream_t s) {
+    const uint32_t nt = G.ntiles;
+    *nbig = 0;
+    if (nt == 0) return GDSJ_OK;
+    // flags[0 .. nt), their exclusive prefix (and the total) behind them
+    uint32_t *pre = flags + nt + 1;
+    k_mj_flags<<<ceil_div(nt, 256), 256, 0, s>>>(G.tiles.as<uint4>(), nt, G.cellStart.as<uint32_t>(), minOcc, flags);
+    GDSJ_TRY(exclusive_scan_u32_u32(flags, pre, nt, true, tmp, s));
+    k_mj_scatter<<<ceil_div(nt, 256), 256, 0, s>>>(flags, pre, nt, list);
+    GDSJ_HIP(hipGetLastError());
+    GDSJ_HIP(hipMemcpyAsync(nbig, pre + nt, 4, hipMemcpyDeviceToHost, s));
+    GDSJ_HIP(hipStreamSynchronize(s));
+    return GDSJ_OK;
 }
 
 const char *join_variant(int dtype, uint32_t d, int mode, uint32_t ilp) {

@@ -35,6 +35,62 @@ struct CMaxFor {
     static constexpr int value = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
 };
 
+// Row stride of Grid::xa for D dims (the grid build uses the same rule).
+template <int D>
+struct XaStride {
+    static constexpr int value = D <= 2 ? 2 : (D <= 4 ? 4 : 8);
+};
+
+// Coordinates of the point at sorted position p from the row-major copy xa
+// (stride XaStride<D>, rows 8-byte aligned, 16-byte aligned from 4 FP32 dims
+// and for FP64): loads of up to 16 bytes that cover only the coordinates,
+// instead of a load per dim from D columns.
+template <int N>
+struct LoadDw;  // N consecutive dwords, 16-byte aligned start
+template <> struct LoadDw<1> { typedef uint32_t v __attribute__((ext_vector_type(1))); };
+template <> struct LoadDw<2> { typedef uint32_t v __attribute__((ext_vector_type(2))); };
+template <> struct LoadDw<3> { typedef uint32_t v __attribute__((ext_vector_type(3))); };
+template <> struct LoadDw<4> { typedef uint32_t v __attribute__((ext_vector_type(4))); };
+
+template <typename T, int D>
+__device__ __forceinline__ void load_row(const T *__restrict__ xa, uint64_t p, T (&c)[D]) {
+    constexpr int S = XaStride<D>::value;
+    constexpr int W = (int)sizeof(T) / 4;  // dwords per coordinate
+    constexpr int NDW = D * W;              // dwords of the row that hold coordinates
+    const uint32_t *row = reinterpret_cast<const uint32_t *>(xa + p * (uint64_t)S);
+    uint32_t dw[NDW];
+    // 16-byte pieces, the last one only as long as the coordinates it holds
+#pragma unroll
+    for (int i = 0; i < NDW; i += 4) {
+        constexpr int dummy = 0;
+        (void)dummy;
+        if (NDW - i >= 4) {
+            const typename LoadDw<4>::v v = *reinterpret_cast<const typename LoadDw<4>::v *>(row + i);
+#pragma unroll
+            for (int k = 0; k < 4; k++) dw[(i + k) < NDW ? i + k : 0] = v[k];
+        } else if (NDW - i == 3) {
+            const typename LoadDw<3>::v v = *reinterpret_cast<const typename LoadDw<3>::v *>(row + i);
+#pragma unroll
+            for (int k = 0; k < 3; k++) dw[(i + k) < NDW ? i + k : 0] = v[k];
+        } else if (NDW - i == 2) {
+            const typename LoadDw<2>::v v = *reinterpret_cast<const typename LoadDw<2>::v *>(row + i);
+#pragma unroll
+            for (int k = 0; k < 2; k++) dw[(i + k) < NDW ? i + k : 0] = v[k];
+        } else {
+            dw[i] = row[i];
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        if constexpr (sizeof(T) == 4) {
+            c[l] = __uint_as_float(dw[l]);
+        } else {
+            const uint64_t b = ((uint64_t)dw[2 * l + 1] << 32) | dw[2 * l];
+            c[l] = __builtin_bit_cast(T, b);
+        }
+    }
+}
+
 // Squared distance in the reference's order.  q: query coords (uniform),
 // c: candidate coords (this lane).  Returns false early (acc > thr on every
 // lane of the wave that is still a candidate) when short-circuiting; the
@@ -305,12 +361,14 @@ __device__ __forceinline__ bool sort_evaluated(const T *__restrict__ xs, uint64_
 // capacity check); a single copy keeps the batch kernel's registers down.
 // BMAP: run-start bitmap for the candidate positions of precomputed runs
 // (else the binary search over the window's runs only).
+// XA: candidates' coordinates from the row-major copy (JoinArgs::xa, which
+// must be set), else from the SoA columns.
 // HM (precomputed runs, no unicomp): hit masks (JoinArgs::hmask).  The
 // counting pass stores each query's chunk ballots; the fill pass takes its
 // hits from them and loads no coordinates.  Both passes cut the candidates
 // into the same windows and chunk batches, so chunk numbers agree.
 template <typename T, int D, int MODE, int ILPV, int PASS, int CM = 0, bool STAMP = false, int QRW = 16,
-          bool UNSW = true, bool BMAP = true, bool HM = false, bool GV = false, bool SRT = false>
+          bool UNSW = true, bool BMAP = true, bool HM = false, bool GV = false, bool SRT = false, bool XA = false>
 __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32_t *__restrict__ ids,
                                           const JoinArgs &a, const uint32_t tile,
                                           const uint64_t *__restrict__ hmr = nullptr) {
@@ -501,7 +559,12 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
                 if (!MFILL && D > 0 && ch < (int)nch) {
                     // lanes past the end read the NaN padding behind the last
                     // point (position ld - 1), which fails every compare
-                    if (off32) {
+                    if (XA) {
+                        T row[D > 0 ? D : 1];
+                        load_row<T, (D > 0 ? D : 1)>((const T *)a.xa, valid[ch] ? (uint64_t)sp : ld - 1, row);
+#pragma unroll
+                        for (int l = 0; l < (D > 0 ? D : 1); l++) c[ch][l] = row[l];
+                    } else if (off32) {
                         const uint32_t bo = (valid[ch] ? sp : (uint32_t)(ld - 1)) * (uint32_t)sizeof(T);
 #pragma unroll
                         for (int l = 0; l < (D > 0 ? D : 1); l++)
@@ -605,7 +668,9 @@ __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const ui
     // fill from its masks cut the candidates into the same chunks
     constexpr int CM = PASS == PASS_SLOT ? CMaxFor<T, GV ? D + 1 : D, GDSJ_CMAX_WORDS_SLOT>::value
                                          : (GV && !HM ? CMaxFor<T, D + 1>::value : 0);
-    join_tile<T, D, MODE, ILPV, PASS, CM, STAMP, 16, true, true, HM, GV, SRT>(xs, ids, a, tile, hm);
+    // up to 8 dims the grid keeps the row-major copy for these kernels
+    constexpr bool XA = D > 0 && D <= 8;
+    join_tile<T, D, MODE, ILPV, PASS, CM, STAMP, 16, true, true, HM, GV, SRT, XA>(xs, ids, a, tile, hm);
 }
 
 }  // namespace gdsj

@@ -105,6 +105,14 @@ typedef struct gdsj_params {
                                reference's CPU paths do, sequentially and unfused
                                (runningDist += t*t, main.cu:963-967; the CLI's mode 0), in
                                place of the kernels' reading of ILP / SHORTCIRCUIT */
+    uint32_t mfma;          /* data whose dims are all indexed (3 to 7 dims): the pair tests of the
+                               tiles of well-occupied cells run on the matrix cores (FP32 MFMA, the
+                               Gram form in tile-local coordinates, concurrently with the plain
+                               kernels on the other tiles); pairs within a proven error band of the
+                               threshold are decided by the reference's arithmetic, so the table is
+                               unchanged (speed only).  0 auto (off: where the pass is bound by
+                               candidate fetches, as on 6-D mixtures, it gains nothing), 1 every
+                               tile, 2 off, n >= 3: the tiles of cells of at least n points */
 } gdsj_params;
 
 /* Timing and diagnostics of the last join on a handle. */
@@ -135,6 +143,7 @@ typedef struct gdsj_stats {
     double ms_train;             /* estimator training on this join's counts */
     double ms_total;             /* sum of the phases above except training */
     uint32_t perm[64];           /* dimension order after REORDER (first 64) */
+    uint64_t mfma_tiles;         /* tiles whose pair tests ran on the matrix cores (gdsj_params.mfma) */
 } gdsj_stats;
 
 typedef struct gdsj_handle gdsj_handle;
