@@ -569,8 +569,8 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
         tq = 64;  // the plain kernels keep one query per lane
     }
     h->G.sortCells = h->p.sort ? 1 : 0;
-    // the row-major copy for the plain kernels' candidate loads (up to 8 dims)
-    h->G.aosWant = !h->use_gram && dims <= 8 ? 1 : 0;
+    // the row-major copy for the plain kernels' candidate loads (up to 16 dims)
+    h->G.aosWant = !h->use_gram && dims <= 16 ? 1 : 0;
     GDSJ_TRY(grid_build(h->G, d_data, n, dims, dtype, epsilon, h->p.k, h->p.reorder, tq, h->p.queryreorder, s));
     if (h->use_gram) {
         GDSJ_TRY(gram_prepare(h->gram, h->G, gram_lowp, s));

@@ -128,6 +128,8 @@ int build_all(gdsj_batch *b, uint32_t tq) {
     for (uint32_t i = 0; i < b->ninst; i++) {
         const gdsj_instance &x = b->in[i];
         Grid &g = *b->G[i];
+        // the specialised bodies (2, 4, 8, 16 dims) read the row-major copy
+        g.aosWant = x.dims == 2 || x.dims == 4 || x.dims == 8 || x.dims == 16;
         GDSJ_TRY(grid_build(g, x.d_data, x.n, x.dims, b->dtype, x.epsilon, b->k[i], b->p.reorder, tq, 0, s));
         bool built = false;
         const bool prune = b->p.prune == 1 || (b->p.prune == 0 && g.k >= 4);
@@ -185,6 +187,8 @@ void make_args(gdsj_batch *b, uint32_t *counts, uint64_t *offsets, uint32_t *ids
         JoinArgs &a = b->hArgs[i];
         a.xs = g.xs.p;
         a.ld = g.ld;
+        a.xa = g.xaStride ? g.xa.p : nullptr;
+        a.xaStride = g.xaStride;
         a.ids = g.ids.as<uint32_t>();
         a.tiles = g.tiles.as<uint4>();
         a.ntiles = g.ntiles;

@@ -68,7 +68,8 @@ struct BatchCMax {
     static constexpr int value = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
 };
 
-// SPEC: specialised bodies for dims 2, 4, 8, 16 (MODE/ILPV fixed), generic
+// SPEC: specialised bodies for dims 2, 4, 8, 16 (MODE/ILPV fixed; they read
+// candidates from the instance's row-major copy, JoinArgs::xa), generic
 // body for the others; !SPEC: generic body only (runtime ILP).
 template <typename T, int MODE, int ILPV, int PASS, bool SPEC, int BUDGET>
 __global__ __launch_bounds__(256) void k_batch_join(const JoinArgs *__restrict__ inst, const BatchLaunch L) {
@@ -80,16 +81,20 @@ __global__ __launch_bounds__(256) void k_batch_join(const JoinArgs *__restrict__
         if (SPEC) {
             switch (a.d) {
                 case 2:
-                    join_tile<T, 2, MODE, ILPV, PASS, BatchCMax<T, 2, BUDGET>::value, false, 0, false, false>(xs, a.ids, a, tile);
+                    join_tile<T, 2, MODE, ILPV, PASS, BatchCMax<T, 2, BUDGET>::value, false, 0, false, false, false, false,
+                              false, true>(xs, a.ids, a, tile);
                     continue;
                 case 4:
-                    join_tile<T, 4, MODE, ILPV, PASS, BatchCMax<T, 4, BUDGET>::value, false, 0, false, false>(xs, a.ids, a, tile);
+                    join_tile<T, 4, MODE, ILPV, PASS, BatchCMax<T, 4, BUDGET>::value, false, 0, false, false, false, false,
+                              false, true>(xs, a.ids, a, tile);
                     continue;
                 case 8:
-                    join_tile<T, 8, MODE, ILPV, PASS, BatchCMax<T, 8, BUDGET>::value, false, 0, false, false>(xs, a.ids, a, tile);
+                    join_tile<T, 8, MODE, ILPV, PASS, BatchCMax<T, 8, BUDGET>::value, false, 0, false, false, false, false,
+                              false, true>(xs, a.ids, a, tile);
                     continue;
                 case 16:
-                    join_tile<T, 16, MODE, ILPV, PASS, BatchCMax<T, 16, BUDGET>::value, false, 0, false, false>(xs, a.ids, a, tile);
+                    join_tile<T, 16, MODE, ILPV, PASS, BatchCMax<T, 16, BUDGET>::value, false, 0, false, false, false, false,
+                              false, true>(xs, a.ids, a, tile);
                     continue;
                 default:
                     break;

@@ -41,6 +41,12 @@ struct GridArgs {
     double prune_lim;      // eps^2 * (1 + margin): squared gap beyond which no pair can match
 };
 
+// Row stride of the row-major copy Grid::xa for d dims: d rounded up to 2,
+// 4, 8 or 16.
+__host__ __device__ constexpr uint32_t xa_stride(uint32_t d) {
+    return d <= 2 ? 2u : (d <= 4 ? 4u : (d <= 8 ? 8u : 16u));
+}
+
 struct Grid {
     // inputs
     uint32_t n = 0, d = 0, k = 0;
@@ -64,11 +70,11 @@ struct Grid {
     // REORDER); posCell maps a sorted position to its cell
     int sortCells = 0;
     DevBuf posCell;
-    // row-major copy of xs (set aosWant before grid_build; up to 8 dims): point
-    // p's coordinates at xa[p * xaStride ..], xaStride = dims rounded up to
-    // 2, 4 or 8, zero-padded, NaN in the padding rows n .. ld-1.  A
-    // candidate's coordinates then share one or two cache lines instead of
-    // one per dim (runs of adjacent cells are short in 6 dims).
+    // row-major copy of xs (set aosWant before grid_build; up to 16 dims):
+    // point p's coordinates at xa[p * xaStride ..], xaStride = dims rounded
+    // up to 2, 4, 8 or 16 (xa_stride), zero-padded, NaN in the padding rows
+    // n .. ld-1.  A candidate's coordinates then share one or two cache lines
+    // instead of one per dim (runs of adjacent cells are short from 6 dims).
     int aosWant = 0;
     uint32_t xaStride = 0;
     DevBuf xa;

@@ -506,7 +506,7 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     uint32_t *nonfinite = G.scalars.as<uint32_t>();
     uint32_t *idsOut = svals == vA ? vB : vA;  // the value buffer the sort did not end in
     GDSJ_HIP(hipMemsetAsync(nonfinite, 0, 4, s));
-    G.xaStride = G.aosWant && d <= 8 ? (d <= 2 ? 2u : (d <= 4 ? 4u : 8u)) : 0u;
+    G.xaStride = G.aosWant && d <= 16 ? xa_stride(d) : 0u;
     T *xa = nullptr;
     if (G.xaStride) {
         GDSJ_TRY(G.xa.ensure((size_t)G.ld * G.xaStride * sizeof(T) + 256));

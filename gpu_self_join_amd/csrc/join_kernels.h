@@ -35,44 +35,47 @@ struct CMaxFor {
     static constexpr int value = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
 };
 
-// Row stride of Grid::xa for D dims (the grid build uses the same rule).
+// Row stride of Grid::xa for D dims (xa_stride, grid.h).
 template <int D>
 struct XaStride {
-    static constexpr int value = D <= 2 ? 2 : (D <= 4 ? 4 : 8);
+    static constexpr int value = (int)xa_stride((uint32_t)D);
 };
 
-// Coordinates of the point at sorted position p from the row-major copy xa
-// (stride XaStride<D>, rows 8-byte aligned, 16-byte aligned from 4 FP32 dims
-// and for FP64): loads of up to 16 bytes that cover only the coordinates,
-// instead of a load per dim from D columns.
 template <int N>
-struct LoadDw;  // N consecutive dwords, 16-byte aligned start
+struct LoadDw;  // N consecutive dwords
 template <> struct LoadDw<1> { typedef uint32_t v __attribute__((ext_vector_type(1))); };
 template <> struct LoadDw<2> { typedef uint32_t v __attribute__((ext_vector_type(2))); };
 template <> struct LoadDw<3> { typedef uint32_t v __attribute__((ext_vector_type(3))); };
 template <> struct LoadDw<4> { typedef uint32_t v __attribute__((ext_vector_type(4))); };
 
+// Coordinates of the point at sorted position p from the row-major copy xa
+// (stride XaStride<D>; rows aligned to the largest power of two dividing
+// their size, at most 16 bytes): loads of up to 16 bytes that cover only the
+// coordinates, instead of a load per dim from D columns.
 template <typename T, int D>
 __device__ __forceinline__ void load_row(const T *__restrict__ xa, uint64_t p, T (&c)[D]) {
     constexpr int S = XaStride<D>::value;
     constexpr int W = (int)sizeof(T) / 4;  // dwords per coordinate
     constexpr int NDW = D * W;              // dwords of the row that hold coordinates
+    constexpr int RB = S * (int)sizeof(T);  // row bytes
+    constexpr int AL = (RB & -RB) < 16 ? (RB & -RB) : 16;
+    constexpr int PW = AL / 4;              // dwords per aligned piece
     const uint32_t *row = reinterpret_cast<const uint32_t *>(xa + p * (uint64_t)S);
     uint32_t dw[NDW];
-    // 16-byte pieces, the last one only as long as the coordinates it holds
 #pragma unroll
-    for (int i = 0; i < NDW; i += 4) {
+    for (int i = 0; i < NDW; i += PW) {
         constexpr int dummy = 0;
         (void)dummy;
-        if (NDW - i >= 4) {
+        const int len = NDW - i < PW ? NDW - i : PW;
+        if (len == 4) {
             const typename LoadDw<4>::v v = *reinterpret_cast<const typename LoadDw<4>::v *>(row + i);
 #pragma unroll
             for (int k = 0; k < 4; k++) dw[(i + k) < NDW ? i + k : 0] = v[k];
-        } else if (NDW - i == 3) {
+        } else if (len == 3) {
             const typename LoadDw<3>::v v = *reinterpret_cast<const typename LoadDw<3>::v *>(row + i);
 #pragma unroll
             for (int k = 0; k < 3; k++) dw[(i + k) < NDW ? i + k : 0] = v[k];
-        } else if (NDW - i == 2) {
+        } else if (len == 2) {
             const typename LoadDw<2>::v v = *reinterpret_cast<const typename LoadDw<2>::v *>(row + i);
 #pragma unroll
             for (int k = 0; k < 2; k++) dw[(i + k) < NDW ? i + k : 0] = v[k];
@@ -668,8 +671,8 @@ __global__ __launch_bounds__(256) void k_join(const T *__restrict__ xs, const ui
     // fill from its masks cut the candidates into the same chunks
     constexpr int CM = PASS == PASS_SLOT ? CMaxFor<T, GV ? D + 1 : D, GDSJ_CMAX_WORDS_SLOT>::value
                                          : (GV && !HM ? CMaxFor<T, D + 1>::value : 0);
-    // up to 8 dims the grid keeps the row-major copy for these kernels
-    constexpr bool XA = D > 0 && D <= 8;
+    // up to 16 dims the grid keeps the row-major copy for these kernels
+    constexpr bool XA = D > 0 && D <= 16;
     join_tile<T, D, MODE, ILPV, PASS, CM, STAMP, 16, true, true, HM, GV, SRT, XA>(xs, ids, a, tile, hm);
 }
 
