@@ -50,7 +50,8 @@ class GdsjParams(C.Structure):
                 ("prune", C.c_uint32), ("train", C.c_uint32), ("filter_stats", C.c_uint32),
                 ("hitmask", C.c_uint32), ("ilp_fold", C.c_uint32),
                 ("vfilter", C.c_uint32), ("sort", C.c_uint32),
-                ("cpu_sum", C.c_uint32), ("mfma", C.c_uint32)]
+                ("cpu_sum", C.c_uint32), ("mfma", C.c_uint32),
+                ("ffilter", C.c_uint32)]
 
 
 class GdsjStats(C.Structure):

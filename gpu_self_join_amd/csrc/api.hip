@@ -251,6 +251,10 @@ static int resolve_params(const gdsj_params *in, uint32_t dims, gdsj_params *out
         set_error("vfilter=%u: 0 auto, 1 on, 2 off", out->vfilter);
         return GDSJ_EINVAL;
     }
+    if (out->ffilter > 2) {
+        set_error("ffilter=%u: 0 auto, 1 on, 2 every pair by the reference's arithmetic", out->ffilter);
+        return GDSJ_EINVAL;
+    }
     if (out->ilp_fold > 1) {
         set_error("ilp_fold=%u: 0 or 1", out->ilp_fold);
         return GDSJ_EINVAL;
@@ -288,6 +292,7 @@ static JoinArgs make_args(gdsj_handle *h) {
     a.ncombo = 1;
     for (uint32_t j = 1; j < h->G.k; j++) a.ncombo *= 3;
     a.thr = h->thr;
+    fma_band(h->G.dtype, h->G.d, h->p.ilp, h->thr, h->p.ffilter, &a.ffLo, &a.ffHi);
     a.d = h->G.d;
     a.ilp = h->p.ilp;
     a.sc = h->p.shortcircuit;

@@ -203,6 +203,7 @@ void make_args(gdsj_batch *b, uint32_t *counts, uint64_t *offsets, uint32_t *ids
         a.thr = b->thr[i];
         a.d = g.d;
         a.ilp = b->tune.ilp;
+        fma_band(g.dtype, g.d, a.ilp, a.thr, b->p.ffilter, &a.ffLo, &a.ffHi);
         a.sc = b->p.shortcircuit;
         a.xcd_map = 0;
         a.prune = b->p.prune == 1 || (b->p.prune == 0 && g.k >= 4) ? 1u : 0u;

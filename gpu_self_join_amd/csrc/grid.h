@@ -12,8 +12,8 @@
 //   cellStart [nc+1]  uint32 sorted-position range of each cell (grid)
 //   cellCoord [k][nc] uint32 cell coordinates per indexed dim
 //   hkeys     [cap][2] uint64 open-addressing table, one 16-byte slot per
-//                     entry: {row (linear id without dim 0) + 1, first cell
-//                     of the row}; or
+//                     row: {row (linear id without dim 0) + 1, first cell of
+//                     the row | number of cells of the row << 32}; or
 //   hvals     [cells] uint32 a direct table linear id -> cell (small grids)
 //                     (both replace the binary searches of kernel.cu:395-398)
 //   tiles     [nt]    uint4  {cell, q0, q1, 0}: up to 64 queries of one cell
@@ -41,11 +41,9 @@ struct GridArgs {
     double prune_lim;      // eps^2 * (1 + margin): squared gap beyond which no pair can match
 };
 
-// Row stride of the row-major copy Grid::xa for d dims: d rounded up to 2,
-// 4, 8 or 16.
-__host__ __device__ constexpr uint32_t xa_stride(uint32_t d) {
-    return d <= 2 ? 2u : (d <= 4 ? 4u : (d <= 8 ? 8u : 16u));
-}
+// Row stride of the row-major copy Grid::xa for d dims: d (no padding, so a
+// run of candidates spans as few cache lines as its coordinates need).
+__host__ __device__ constexpr uint32_t xa_stride(uint32_t d) { return d; }
 
 struct Grid {
     // inputs
@@ -71,9 +69,8 @@ struct Grid {
     int sortCells = 0;
     DevBuf posCell;
     // row-major copy of xs (set aosWant before grid_build; up to 16 dims):
-    // point p's coordinates at xa[p * xaStride ..], xaStride = dims rounded
-    // up to 2, 4, 8 or 16 (xa_stride), zero-padded, NaN in the padding rows
-    // n .. ld-1.  A candidate's coordinates then share one or two cache lines
+    // point p's coordinates at xa[p * xaStride ..], xaStride = dims
+    // (xa_stride), NaN in the padding rows n .. ld-1.  A candidate's coordinates then share one or two cache lines
     // instead of one per dim (runs of adjacent cells are short from 6 dims).
     int aosWant = 0;
     uint32_t xaStride = 0;
@@ -131,6 +128,17 @@ __device__ __forceinline__ int32_t cell_find(const uint64_t *__restrict__ hkeys,
 
 // first index i in [lo, hi) with keys[i] >= want (hi if none); galloping
 // from lo, so a short distance costs few loads
+// First index in [lo, hi) of a sorted uint32 range whose value is >= want
+// (hi if none).
+__device__ __forceinline__ uint32_t u32_lower_bound(const uint32_t *__restrict__ v, uint32_t lo, uint32_t hi,
+                                                    uint32_t want) {
+    while (lo < hi) {
+        const uint32_t m = lo + (hi - lo) / 2;
+        if (v[m] < want) lo = m + 1; else hi = m;
+    }
+    return lo;
+}
+
 __host__ __device__ __forceinline__ uint32_t key_lower_bound(const uint64_t *__restrict__ keys, uint32_t lo, uint32_t hi,
                                                     uint64_t want) {
     uint32_t step = 1, a = lo, b = lo;

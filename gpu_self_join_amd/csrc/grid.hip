@@ -267,20 +267,23 @@ __global__ __launch_bounds__(256) void k_soa_tile(const T *__restrict__ data, ui
 // Open-addressing table row -> first cell of the row, the replacement for
 // the per-candidate-cell binary search of the reference (kernel.cu:388-399).
 // Rows: cells equal in dims 1..k-1 (linear id minus c_0); each row's first
-// cell is inserted under row + 1.
+// cell c is inserted under row + 1 with value c | (cells in the row) << 32
+// (a row's cells are contiguous in cell order).
 __global__ void k_hash_build(const uint64_t *__restrict__ cellKey, uint32_t nc, uint32_t ncells0, uint32_t bits,
                              unsigned long long *__restrict__ hkeys) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const uint64_t row = cellKey[c] - cellKey[c] % ncells0;
     if (c > 0 && cellKey[c - 1] - cellKey[c - 1] % ncells0 == row) return;  // not the row's first cell
+    uint32_t len = 1;
+    while (c + len < nc && cellKey[c + len] < row + ncells0) len++;
     const uint32_t mask = (1u << bits) - 1u;
     const unsigned long long key = (unsigned long long)row + 1ull;
     uint32_t h = cell_hash(row, bits);
     for (uint32_t probe = 0; probe <= mask; probe++) {
         unsigned long long prev = atomicCAS(&hkeys[2 * (uint64_t)h], 0ull, key);
         if (prev == 0ull || prev == key) {
-            hkeys[2 * (uint64_t)h + 1] = c;
+            hkeys[2 * (uint64_t)h + 1] = (unsigned long long)c | ((unsigned long long)len << 32);
             return;
         }
         h = (h + 1) & mask;

@@ -52,6 +52,13 @@ struct JoinArgs {
     GridArgs ga;
     uint32_t ncombo;   // 3^(k-1) neighbor combinations of dims 1..k-1
     double thr;        // largest DTYPE value x with sqrt(x) <= eps (exact in double)
+    // FMA-chain filter of the plain kernels' register-resident pair tests
+    // (ILP > 0, counting passes): f = the squared differences summed by one
+    // FMA each decides
+    // a pair when f <= ffLo (a neighbor) or f > ffHi (not one); a chunk with
+    // a pair in between is evaluated in the reference's order (fma_band).
+    // Zero-initialised args are unsafe here: every JoinArgs gets its band.
+    double ffLo, ffHi;
     // Gram-form filter of the plain kernels (every dim indexed): a pair is a
     // neighbor when |c'|^2 - 2 q'.c' <= gvLo - |q'|^2 and not one when it
     // exceeds gvHi - |q'|^2 (c', q': coordinates relative to the tile's
@@ -149,6 +156,10 @@ bool gram_valu_band(const Grid &G, double thr, float *lo, float *mid, float *hal
 // False when it does not apply (other dims, cells too fine for float
 // coordinates, a band wider than 5 % of thr).
 bool mjoin_band(const Grid &G, double thr, float *hi, float *lo);
+// The FMA-chain filter's band for DTYPE dtype, d dims and ILP ilp; mode 2
+// gives a band that holds every pair (each decided by the reference's
+// arithmetic).
+void fma_band(int dtype, uint32_t d, uint32_t ilp, double thr, uint32_t mode, double *lo, double *hi);
 
 // list[0 .. *nbig) = the tiles whose cell holds at least minOcc points,
 // list[*nbig .. ntiles) = the others, both in tile order (tmp: scan scratch

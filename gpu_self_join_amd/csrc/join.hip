@@ -116,6 +116,44 @@ bool mjoin_band(const Grid &G, double thr, float *hi, float *lo) {
     return true;
 }
 
+// The FMA-chain filter's band.  Both the filter's f (a product, then one
+// FMA per dim: every term t_j^2 of S = sum t_j^2, t_j = fl(c_j - q_j), meets
+// at most n = d + ilp roundings) and the reference's acc (kernel.cu:307-357:
+// products and adds, fused first dims, ILP partials and their fold; again at
+// most n roundings per term) satisfy |x - S| <= g S + n eta, g = gamma(n),
+// eta the spacing of subnormals, so
+//   f <= lo = (thr - n eta)(1 - g)/(1 + g) - n eta  implies acc <= thr,
+//   f >  hi = (thr + n eta)(1 + g)/(1 - g) + n eta  implies acc >  thr.
+// g is doubled (plus 8 u) and lo / hi are rounded outward to DTYPE for the
+// host's own roundings.  Partial sums only grow in the reference's arithmetic, so a
+// partial f above hi also decides the early exit of SHORTCIRCUIT.
+void fma_band(int dtype, uint32_t d, uint32_t ilp, double thr, uint32_t mode, double *lo, double *hi) {
+    if (mode == 2 || !(thr >= 0.0) || !(thr < 1e300)) {
+        // every pair goes to the reference's arithmetic
+        *lo = -1.0;
+        *hi = __builtin_inf();
+        return;
+    }
+    const bool f32 = dtype == GDSJ_DTYPE_F32;
+    const double u = f32 ? 0x1p-24 : 0x1p-53, eta = f32 ? 0x1p-149 : 0x1p-1074;
+    const double n = (double)d + (double)ilp + 1.0;
+    const double g = 2.0 * (n * u / (1.0 - n * u)) + 8.0 * u;
+    double l = (thr - n * eta) * (1.0 - g) / (1.0 + g) - n * eta;
+    double h = (thr + n * eta) * (1.0 + g) / (1.0 - g) + n * eta;
+    if (f32) {
+        float lf = (float)l, hf = (float)h;
+        if ((double)lf > l) lf = nextafterf(lf, -INFINITY);
+        if ((double)hf < h) hf = nextafterf(hf, INFINITY);
+        l = lf;
+        h = hf;
+    } else {
+        l = nextafter(l, -INFINITY);
+        h = nextafter(h, INFINITY);
+    }
+    *lo = l < 0.0 ? -1.0 : l;
+    *hi = h;
+}
+
 __global__ void k_mj_flags(const uint4 *__restrict__ tiles, uint32_t nt, const uint32_t *__restrict__ cellStart,
                            uint32_t minOcc, uint32_t *__restrict__ flags) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;

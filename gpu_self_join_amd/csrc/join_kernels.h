@@ -49,34 +49,32 @@ template <> struct LoadDw<3> { typedef uint32_t v __attribute__((ext_vector_type
 template <> struct LoadDw<4> { typedef uint32_t v __attribute__((ext_vector_type(4))); };
 
 // Coordinates of the point at sorted position p from the row-major copy xa
-// (stride XaStride<D>; rows aligned to the largest power of two dividing
-// their size, at most 16 bytes): loads of up to 16 bytes that cover only the
-// coordinates, instead of a load per dim from D columns.
+// (stride XaStride<D> = D, so rows are only dword-aligned): loads of up to
+// 16 bytes that cover the row (the hardware takes multi-dword loads at any
+// dword alignment), instead of a load per dim from D columns.
 template <typename T, int D>
 __device__ __forceinline__ void load_row(const T *__restrict__ xa, uint64_t p, T (&c)[D]) {
     constexpr int S = XaStride<D>::value;
     constexpr int W = (int)sizeof(T) / 4;  // dwords per coordinate
-    constexpr int NDW = D * W;              // dwords of the row that hold coordinates
-    constexpr int RB = S * (int)sizeof(T);  // row bytes
-    constexpr int AL = (RB & -RB) < 16 ? (RB & -RB) : 16;
-    constexpr int PW = AL / 4;              // dwords per aligned piece
+    constexpr int NDW = D * W;              // dwords of the row
     const uint32_t *row = reinterpret_cast<const uint32_t *>(xa + p * (uint64_t)S);
     uint32_t dw[NDW];
 #pragma unroll
-    for (int i = 0; i < NDW; i += PW) {
-        constexpr int dummy = 0;
-        (void)dummy;
-        const int len = NDW - i < PW ? NDW - i : PW;
+    for (int i = 0; i < NDW; i += 4) {
+        const int len = NDW - i < 4 ? NDW - i : 4;
         if (len == 4) {
-            const typename LoadDw<4>::v v = *reinterpret_cast<const typename LoadDw<4>::v *>(row + i);
+            typename LoadDw<4>::v v;
+            __builtin_memcpy(&v, row + i, 16);
 #pragma unroll
             for (int k = 0; k < 4; k++) dw[(i + k) < NDW ? i + k : 0] = v[k];
         } else if (len == 3) {
-            const typename LoadDw<3>::v v = *reinterpret_cast<const typename LoadDw<3>::v *>(row + i);
+            typename LoadDw<3>::v v;
+            __builtin_memcpy(&v, row + i, 12);
 #pragma unroll
             for (int k = 0; k < 3; k++) dw[(i + k) < NDW ? i + k : 0] = v[k];
         } else if (len == 2) {
-            const typename LoadDw<2>::v v = *reinterpret_cast<const typename LoadDw<2>::v *>(row + i);
+            typename LoadDw<2>::v v;
+            __builtin_memcpy(&v, row + i, 8);
 #pragma unroll
             for (int k = 0; k < 2; k++) dw[(i + k) < NDW ? i + k : 0] = v[k];
         } else {
@@ -157,6 +155,30 @@ __device__ __forceinline__ T dist_reg(const T (&q)[D], const T (&c)[D], T thr, b
         for (int j = 1; j < ILPV; j++) acc = acc + r[j];
         return acc;
     }
+}
+
+// The FMA-chain filter value of a pair: t_0^2, then one FMA per dim (2 D
+// operations against the reference's 3 D - 1 with ILP), compared with the
+// band of JoinArgs::ffLo / ffHi by the caller.  In dist_reg's ILP_SC order
+// the wave leaves early (alive = false) after a group of ILPV dims when no
+// live lane's partial sum is at most hi: the reference's partial sums then
+// exceed the threshold, and they only grow.
+template <typename T, int D, int MODE, int ILPV>
+__device__ __forceinline__ T fma_chain(const T (&q)[D], const T (&c)[D], T hi, bool live, bool sc, bool &alive) {
+    alive = true;
+    T f = 0;
+#pragma unroll
+    for (int l = 0; l < D; l++) {
+        const T t = c[l] - q[l];
+        f = fma_rn(t, t, f);
+        if (MODE == ACC_ILP_SC && sc && (l % ILPV) == ILPV - 1 && l + 1 < D) {
+            if (ballot(live && f <= hi) == 0) {
+                alive = false;
+                return f;
+            }
+        }
+    }
+    return f;
 }
 
 // Same contract with runtime dims and ILP; coordinates read from memory.
@@ -387,6 +409,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
     const bool off32 = D > 0 && (uint64_t)(D > 0 ? D : 1) * ld * sizeof(T) <= 0xFFFFFFFFull;
     const __amdgpu_buffer_rsrc_t xrs = buffer_rsrc(xs);
     const T thr = (T)a.thr;
+    const T ffl = (T)a.ffLo, ffh = (T)a.ffHi;  // FMA-chain filter band (fma_band)
     const uint4 tl = a.tiles[tile];
     const uint32_t cell = uniform(tl.x), q0 = uniform(tl.y), nq = uniform(tl.z - tl.y);
     if (nq == 0) return;  // an empty sub-tile (split overflowed tiles)

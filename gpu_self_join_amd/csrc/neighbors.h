@@ -127,81 +127,104 @@ __device__ __forceinline__ uint2 combo_run(uint32_t t, const uint32_t (&cc)[MAXK
 }
 
 // The runs of R decoded rows w[0..R) (combo_run's lookup half): each
-// lookup step (row probe, row keys, cell starts) issues its loads for all R
+// lookup step (row probe, row cells, cell starts) issues its loads for all R
 // before the first of them is used, so a wave waits for R dependent chains
-// together instead of one after another.
+// together instead of one after another.  Hashed grids: the row's slot gives
+// its first cell and its length; the dim-0 coordinates of the row's first 8
+// cells (cellCoord's dim-0 slice, sorted within the row) come in two 16-byte
+// loads and the run's bounds are counts over them; rows past 8 cells continue
+// by a binary search.
 template <int R>
 __device__ __forceinline__ void combo_lookup(const ComboRow (&w)[R], const GridArgs &ga,
                                              const uint32_t *__restrict__ cellStart,
-                                             const uint64_t *__restrict__ cellKey,
+                                             const uint32_t *__restrict__ cellC0,
                                              const uint64_t *__restrict__ hkeys,
                                              const uint32_t *__restrict__ hvals, uint2 (&run)[R]) {
 #pragma unroll
     for (int r = 0; r < R; r++) run[r] = make_uint2(0u, 0u);
     int32_t first[R], last[R];  // cell range of the run, -1: none
     if (ga.hbits) {
-        // first probe of every row (a table at most half full mostly
-        // answers there); the rare longer probe sequences continue below
         const uint32_t mask = (1u << ga.hbits) - 1u;
-        uint32_t h[R];
-        uint64_t key[R];
-        uint32_t val[R];
+        const ulonglong2 *slots = reinterpret_cast<const ulonglong2 *>(hkeys);
+        uint32_t h[R], len[R];
+        ulonglong2 e[R];
+        bool pend[R];
+        // first probe of every row (a table at most half full mostly
+        // answers there)
 #pragma unroll
         for (int r = 0; r < R; r++) {
             h[r] = cell_hash(w[r].base, ga.hbits);
-            key[r] = 0;
-            val[r] = 0;
-            if (w[r].live) {
-                const ulonglong2 e = reinterpret_cast<const ulonglong2 *>(hkeys)[h[r]];
-                key[r] = e.x;
-                val[r] = (uint32_t)e.y;
-            }
+            e[r] = make_ulonglong2(0ull, 0ull);
+            if (w[r].live) e[r] = slots[h[r]];
         }
+        // the longer probe sequences, all rows of the wave in lock step (a
+        // wave waits for its longest sequence once, not once per r); every
+        // sequence ends at the row's slot or at an empty slot, and at least
+        // half of the slots are empty
+        for (uint32_t probe = 0;; probe++) {
+            bool any = false;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                first[r] = -1;
+                len[r] = 0;
+                pend[r] = false;
+                if (!w[r].live || e[r].x == 0) continue;
+                if (e[r].x == w[r].base + 1) {
+                    first[r] = (int32_t)(uint32_t)e[r].y;
+                    len[r] = (uint32_t)(e[r].y >> 32);
+                    continue;
+                }
+                pend[r] = true;
+                any = true;
+            }
+            if (!ballot(any) || probe > mask) break;
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                if (pend[r]) {
+                    h[r] = (h[r] + 1u) & mask;
+                    e[r] = slots[h[r]];
+                }
+        }
+        // dim-0 coordinates of the row's first 8 cells (sorted); the slice
+        // is followed by the other dims' and 64 bytes of padding, so the
+        // loads stay inside the buffer
+        uint32_t cv[R][8];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            first[r] = -1;
-            if (!w[r].live || key[r] == 0) continue;
-            if (key[r] == w[r].base + 1) {
-                first[r] = (int32_t)val[r];
-                continue;
-            }
-            uint32_t hh = (h[r] + 1) & mask;
-            for (uint32_t probe = 1; probe <= mask; probe++) {
-                const ulonglong2 e = reinterpret_cast<const ulonglong2 *>(hkeys)[hh];
-                if (e.x == w[r].base + 1) {
-                    first[r] = (int32_t)e.y;
-                    break;
+#pragma unroll
+            for (int i = 0; i < 8; i++) cv[r][i] = 0xFFFFFFFFu;
+            if (first[r] >= 0) {
+                uint4 v;
+                __builtin_memcpy(&v, cellC0 + first[r], 16);
+                cv[r][0] = v.x;
+                cv[r][1] = v.y;
+                cv[r][2] = v.z;
+                cv[r][3] = v.w;
+                if (len[r] > 4) {
+                    __builtin_memcpy(&v, cellC0 + first[r] + 4, 16);
+                    cv[r][4] = v.x;
+                    cv[r][5] = v.y;
+                    cv[r][6] = v.z;
+                    cv[r][7] = v.w;
                 }
-                if (e.x == 0) break;
-                hh = (hh + 1) & mask;
             }
         }
-        // the row's first four keys, for all rows
-        uint64_t kk[R][4];
-#pragma unroll
-        for (int r = 0; r < R; r++)
-#pragma unroll
-            for (uint32_t i = 0; i < 4; i++) {
-                kk[r][i] = ~0ull;
-                if (first[r] >= 0 && (uint32_t)first[r] + i < ga.ncell) kk[r][i] = cellKey[(uint32_t)first[r] + i];
-            }
 #pragma unroll
         for (int r = 0; r < R; r++) {
             if (first[r] < 0) continue;
-            const uint32_t rr = (uint32_t)first[r];
-            const uint64_t w0 = w[r].base + w[r].x0, w1 = w[r].base + w[r].x1 + 1;
+            const uint32_t rr = (uint32_t)first[r], n8 = len[r] < 8u ? len[r] : 8u;
             uint32_t nf = 0, ne = 0;
 #pragma unroll
-            for (uint32_t i = 0; i < 4; i++) {
-                nf += kk[r][i] < w0 ? 1u : 0u;
-                ne += kk[r][i] < w1 ? 1u : 0u;
+            for (uint32_t i = 0; i < 8; i++) {
+                nf += i < n8 && cv[r][i] < w[r].x0 ? 1u : 0u;
+                ne += i < n8 && cv[r][i] <= w[r].x1 ? 1u : 0u;
             }
-            uint32_t f = rr + nf, e = rr + ne;
-            if (nf == 4) f = key_lower_bound(cellKey, rr + 4, ga.ncell, w0);
-            if (ne == 4) e = key_lower_bound(cellKey, f, ga.ncell, w1);
+            uint32_t f = rr + nf, en = rr + ne;
+            if (nf == 8u) f = u32_lower_bound(cellC0, rr + 8u, rr + len[r], w[r].x0);
+            if (ne == 8u) en = u32_lower_bound(cellC0, f > rr + 8u ? f : rr + 8u, rr + len[r], w[r].x1 + 1u);
             // as a cell range [first, last]
-            first[r] = f < e ? (int32_t)f : -1;
-            last[r] = (int32_t)e - 1;
+            first[r] = f < en ? (int32_t)f : -1;
+            last[r] = (int32_t)en - 1;
         }
     } else {
         // direct table: the up to three cells of every row

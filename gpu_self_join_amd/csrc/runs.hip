@@ -127,10 +127,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R > 1 ? 6 :
     uint2 *mine = runs + (uint64_t)cell * stride;
     for (uint32_t cw = 0; cw < ncombo; cw += GDSJ_WAVE * R) {
         ComboRow w[R];
+        uint4 te[R];  // table entries, loaded together
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const uint32_t t = cw + (uint32_t)r * GDSJ_WAVE + lane;
-            const uint4 e = tab[t < ncombo ? t : 0u];
+            te[r] = tab[t < ncombo ? t : 0u];
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t t = cw + (uint32_t)r * GDSJ_WAVE + lane;
+            const uint4 e = te[r];
             const double g = gA[e.z & 0xFFFu] + gB[(e.z >> 12) & 0xFFFu];
             const uint32_t top = (e.z >> 24) & 0xFu, topd = e.z >> 28;
             // unicomp ownership (combo_decode's rules)
@@ -146,7 +152,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R > 1 ? 6 :
             if (stamp == 2 && top == 0) w[r].x0 = c0;
         }
         uint2 rr[R];
-        combo_lookup<R>(w, ga, cellStart, cellKey, hkeys, hvals, rr);
+        combo_lookup<R>(w, ga, cellStart, cellCoord, hkeys, hvals, rr);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const uint2 run = rr[r];

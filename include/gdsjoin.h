@@ -113,6 +113,13 @@ typedef struct gdsj_params {
                                unchanged (speed only).  0 auto (off: where the pass is bound by
                                candidate fetches, as on 6-D mixtures, it gains nothing), 1 every
                                tile, 2 off, n >= 3: the tiles of cells of at least n points */
+    uint32_t ffilter;       /* ILP > 0, counting passes of the plain kernels (register-resident
+                               pair tests): each pair's squared differences are first summed by
+                               one FMA per dim and compared with a proven error band around the
+                               threshold; a 64-pair chunk with a pair inside the band is
+                               evaluated in the reference's order, so the counts are unchanged
+                               (speed only).  0 auto (on), 1 on, 2 every pair in the band (the
+                               reference's arithmetic for all) */
 } gdsj_params;
 
 /* Timing and diagnostics of the last join on a handle. */
