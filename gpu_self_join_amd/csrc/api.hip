@@ -251,6 +251,10 @@ static int resolve_params(const gdsj_params *in, uint32_t dims, gdsj_params *out
         set_error("vfilter=%u: 0 auto, 1 on, 2 off", out->vfilter);
         return GDSJ_EINVAL;
     }
+    if (out->index > 2) {
+        set_error("index=%u: 0 auto, 1 the reference's dims, 2 principal axes", out->index);
+        return GDSJ_EINVAL;
+    }
     if (out->ffilter > 2) {
         set_error("ffilter=%u: 0 auto, 1 on, 2 every pair by the reference's arithmetic", out->ffilter);
         return GDSJ_EINVAL;
@@ -276,6 +280,7 @@ static JoinArgs make_args(gdsj_handle *h) {
     JoinArgs a{};
     a.xs = h->G.xs.p;
     a.ld = h->G.ld;
+    a.ixs = h->G.indexUsed == 2 ? h->G.ixs.p : h->G.xs.p;
     if (h->G.xaStride) {
         a.xa = h->G.xa.p;
         a.xaStride = h->G.xaStride;
@@ -574,6 +579,10 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
         tq = 64;  // the plain kernels keep one query per lane
     }
     h->G.sortCells = h->p.sort ? 1 : 0;
+    // the index: the reference's dims, or the principal axes (not with
+    // unicomp, whose ownership rules are stated on the reference's cells)
+    h->G.indexWant = h->stamp ? 1 : (h->p.index == 0 ? 0 : (int)h->p.index);
+    h->G.ilpTerms = h->p.ilp;
     // the row-major copy for the plain kernels' candidate loads (up to 16 dims)
     h->G.aosWant = !h->use_gram && dims <= 16 ? 1 : 0;
     GDSJ_TRY(grid_build(h->G, d_data, n, dims, dtype, epsilon, h->p.k, h->p.reorder, tq, h->p.queryreorder, s));
@@ -702,6 +711,8 @@ static int join_passes(gdsj_handle *h, uint32_t n, uint32_t dims, uint32_t *d_co
     h->st.candidates = hres[1];
     h->st.n_nonempty_cells = h->G.ncell;
     h->st.total_cells = h->G.totalCells;
+    h->st.index_used = (uint64_t)h->G.indexUsed;
+    h->st.index_risky = h->G.riskyPoints;
     h->st.n_tiles = h->G.ntiles;
     h->st.n_runs = h->G.haveRuns ? 0 : (uint64_t)h->G.ntiles * a.ncombo;
     h->st.est_overflow_tiles = h->novf;

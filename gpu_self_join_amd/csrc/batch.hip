@@ -187,6 +187,7 @@ void make_args(gdsj_batch *b, uint32_t *counts, uint64_t *offsets, uint32_t *ids
         JoinArgs &a = b->hArgs[i];
         a.xs = g.xs.p;
         a.ld = g.ld;
+        a.ixs = g.xs.p;
         a.xa = g.xaStride ? g.xa.p : nullptr;
         a.xaStride = g.xaStride;
         a.ids = g.ids.as<uint32_t>();

@@ -75,6 +75,21 @@ struct Grid {
     int aosWant = 0;
     uint32_t xaStride = 0;
     DevBuf xa;
+    // index (set indexWant and ilpTerms before grid_build): 1 the reference's
+    // k dims of largest variance; 2 the top k principal axes of the REORDER
+    // sample (y = R x, cells of width cellW > eps so that every pair the
+    // reference's arithmetic accepts lies in adjacent cells), used only when
+    // no point lies within rounding of a face of the reference's grid
+    // (riskyPoints = 0); 0 auto (principal axes when they leave at most a
+    // twentieth of the variance the reference's dims leave).  ixs: the index
+    // coordinates in sorted order ([k][ld], principal axes only), which the
+    // run pruning reads instead of xs; ybuf: the same in input order.
+    int indexWant = 1;
+    uint32_t ilpTerms = 0;
+    int indexUsed = 1;
+    uint64_t riskyPoints = 0;
+    double cellW = 0;
+    DevBuf ixs, ybuf, pcaBuf;
     DevBuf keyA, keyB, valB, tmp, aux, flag, tileKey, tileVal, cellAux, scalars;
     HostBuf hostSample;  // REORDER's sample rows, pinned (the variance is summed on the host)
     void release();

@@ -159,6 +159,157 @@ __global__ void k_cell_keys(const T *__restrict__ data, uint32_t n, uint32_t d, 
     if (bad) atomicOr(outside, 1u);
 }
 
+// ------------------------------------------------ principal-axes index --
+// An index on the principal axes of the data (Grid::indexWant) instead of
+// the reference's k dims of largest variance: the candidate set changes, the
+// pair test does not.  The grid is built on y = R x (R: k orthonormal rows,
+// the top principal directions of the REORDER sample) with cells wide
+// enough that every pair the reference's arithmetic accepts lies in adjacent
+// cells, and it is used only when no point sits within rounding of a face of
+// the reference's own grid, so that the reference's candidate set holds
+// every accepted pair as well: both tables are then the set of pairs the
+// reference's arithmetic accepts.
+
+// covariance of the m sample rows about mean (rows row-major, d columns);
+// one block per (i, j >= i), written to both halves
+template <typename T>
+__global__ __launch_bounds__(256) void k_cov(const T *__restrict__ samp, uint32_t m, uint32_t d,
+                                             const double *__restrict__ mean, double *__restrict__ cov) {
+    const uint32_t i = blockIdx.y, j = blockIdx.x;
+    if (j < i) return;  // uniform per block
+    double acc = 0.0;
+    for (uint32_t p = threadIdx.x; p < m; p += blockDim.x)
+        acc += ((double)samp[(uint64_t)p * d + i] - mean[i]) * ((double)samp[(uint64_t)p * d + j] - mean[j]);
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cov[(uint64_t)i * d + j] = red[0];
+        cov[(uint64_t)j * d + i] = red[0];
+    }
+}
+
+// Points whose quotient q = (x - min) / eps, computed as k_cell_keys computes
+// it, lies within tau of an integer in one of the reference's indexed dims:
+// list entry (dim << 56 | below << 55 | floor(q)), below = q less than tau
+// under the next integer, else q at most tau above floor(q).  Two points
+// whose quotients differ by at most 1 + tau in a dim fall two cells apart
+// there only as such a pair: q_a < m + 1 and q_b >= m + 2 give q_a >= m + 1
+// - tau (a below, floor m) and q_b < m + 2 + tau (b above, floor m + 2).
+template <typename T>
+__global__ void k_face_risk(const T *__restrict__ data, uint32_t n, uint32_t d, KeyArgs a, T eps, double tau,
+                            uint32_t cap, uint32_t *__restrict__ count, uint64_t *__restrict__ list) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (uint32_t j = 0; j < a.k; j++) {
+        const T x = data[(uint64_t)i * d + a.pk[j]];
+        const T q = (x - (T)a.minArr[j]) / eps;
+        const double qd = (double)q, fl = floor(qd), f = qd - fl;
+        const bool below = !(f < 1.0 - tau), above = f < tau;
+        if (!below && !above) continue;
+        const uint32_t slot = atomicAdd(count, 1u);
+        if (slot < cap)
+            list[slot] = ((uint64_t)j << 56) | ((uint64_t)(below ? 1u : 0u) << 55) | (uint64_t)fmax(fl, 0.0);
+    }
+}
+
+// y = R x for every point (R: k x d doubles, summed in double by FMAs,
+// stored in DTYPE), and the largest |x|^2 (double bits, non-negative, so
+// their integer order is their value order)
+template <typename T>
+__global__ __launch_bounds__(256) void k_project(const T *__restrict__ data, uint32_t n, uint32_t d, uint32_t k,
+                                                 const double *__restrict__ R, T *__restrict__ y,
+                                                 unsigned long long *__restrict__ norm2max) {
+    extern __shared__ double sR[];
+    for (uint32_t t = threadIdx.x; t < k * d; t += blockDim.x) sR[t] = R[t];
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double nn = 0.0;
+    if (i < n) {
+        double acc[MAXK];
+#pragma unroll
+        for (int c = 0; c < MAXK; c++) acc[c] = 0.0;
+        for (uint32_t j = 0; j < d; j++) {
+            const double x = (double)data[(uint64_t)i * d + j];
+            nn = fma(x, x, nn);
+#pragma unroll
+            for (int c = 0; c < MAXK; c++)
+                if ((uint32_t)c < k) acc[c] = fma(sR[(uint32_t)c * d + j], x, acc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < MAXK; c++)
+            if ((uint32_t)c < k) y[(uint64_t)i * k + c] = (T)acc[c];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) nn = fmax(nn, __shfl_xor(nn, off, GDSJ_WAVE));
+    if (lane_id() == 0) atomicMax(norm2max, (unsigned long long)__double_as_longlong(nn));
+}
+
+// SoA copy of the index coordinates in sorted order (NaN past n)
+template <typename T>
+__global__ void k_gather_index(const T *__restrict__ y, uint32_t n, uint32_t k, const uint32_t *__restrict__ order,
+                               uint64_t ld, T *__restrict__ ixs) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ld) return;
+    for (uint32_t c = 0; c < k; c++)
+        ixs[(uint64_t)c * ld + p] = p < n ? y[(uint64_t)order[p] * k + c] : (T)__builtin_nan("");
+}
+
+// The top-k principal directions of the covariance C (d x d) by subspace
+// iteration from the unit vectors of the k dims of largest variance
+// (perm[0..k)), modified Gram-Schmidt after every product.  R (k x d, rows
+// orthonormal up to rounding); *capR / *capP: variance captured by those k
+// dims / by R; *total: trace of C.
+static void pca_basis(const std::vector<double> &C, uint32_t d, uint32_t k, const uint32_t *perm,
+                      std::vector<double> &R, double *capR, double *capP, double *total) {
+    std::vector<double> Q((size_t)d * k, 0.0), Z((size_t)d * k);
+    for (uint32_t c = 0; c < k; c++) Q[(size_t)perm[c] * k + c] = 1.0;
+    auto orth = [&](std::vector<double> &M) {
+        for (uint32_t c = 0; c < k; c++) {
+            for (uint32_t b = 0; b < c; b++) {
+                double dot = 0.0;
+                for (uint32_t j = 0; j < d; j++) dot += M[(size_t)j * k + c] * M[(size_t)j * k + b];
+                for (uint32_t j = 0; j < d; j++) M[(size_t)j * k + c] -= dot * M[(size_t)j * k + b];
+            }
+            double nn = 0.0;
+            for (uint32_t j = 0; j < d; j++) nn += M[(size_t)j * k + c] * M[(size_t)j * k + c];
+            const double inv = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+            for (uint32_t j = 0; j < d; j++) M[(size_t)j * k + c] *= inv;
+            if (nn == 0.0) M[(size_t)perm[c] * k + c] = 1.0;  // degenerate: keep the unit vector
+        }
+    };
+    for (int it = 0; it < 12; it++) {
+        for (uint32_t i = 0; i < d; i++)
+            for (uint32_t c = 0; c < k; c++) {
+                double acc = 0.0;
+                for (uint32_t j = 0; j < d; j++) acc += C[(size_t)i * d + j] * Q[(size_t)j * k + c];
+                Z[(size_t)i * k + c] = acc;
+            }
+        orth(Z);
+        Q.swap(Z);
+    }
+    R.assign((size_t)k * d, 0.0);
+    *capP = 0.0;
+    for (uint32_t c = 0; c < k; c++) {
+        for (uint32_t j = 0; j < d; j++) R[(size_t)c * d + j] = Q[(size_t)j * k + c];
+        double q = 0.0;
+        for (uint32_t i = 0; i < d; i++) {
+            double row = 0.0;
+            for (uint32_t j = 0; j < d; j++) row += C[(size_t)i * d + j] * Q[(size_t)j * k + c];
+            q += Q[(size_t)i * k + c] * row;
+        }
+        *capP += q;
+    }
+    *total = 0.0;
+    for (uint32_t j = 0; j < d; j++) *total += C[(size_t)j * d + j];
+    *capR = 0.0;
+    for (uint32_t c = 0; c < k; c++) *capR += C[(size_t)perm[c] * d + perm[c]];
+}
+
 // SORT: the cell of every sorted position (inclusive count of cell heads - 1)
 __global__ void k_pos_cell(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ cidx, uint32_t n,
                            uint32_t *__restrict__ posCell) {
@@ -352,7 +503,7 @@ __global__ void k_tiles_permute(const uint4 *__restrict__ in, const uint32_t *__
 void Grid::release() {
     DevBuf *b[] = {&xs, &ids, &cellKey, &cellStart, &cellCoord, &hkeys, &hvals, &tiles, &keyA, &keyB,
                    &valB, &tmp, &aux, &flag, &tileKey, &tileVal, &cellAux, &scalars, &runs, &nruns, &cellCand, &cellHist,
-                   &comboTab, &posCell, &xa};
+                   &comboTab, &posCell, &xa, &ixs, &ybuf, &pcaBuf};
     for (DevBuf *p : b) p->release();
     hostSample.release();
 }
@@ -367,7 +518,7 @@ static int bit_length(uint64_t v) {
 template <typename T>
 static int host_extent(Grid &G, const T *mn, const T *mx) {
     uint64_t total = 1;
-    T eps = (T)G.eps;
+    T eps = (T)G.cellW;
     for (uint32_t j = 0; j < G.k; j++) {
         T lo = mn[j] - eps;
         T hi = mx[j] + eps;
@@ -423,6 +574,55 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
             G.perm[j + 1] = pi;
         }
     }
+    // ---- principal-axes index: the basis from the REORDER sample rows
+    bool pca = false;
+    std::vector<double> R;
+    double delta = 0.0;
+    G.cellW = G.eps;
+    G.indexUsed = 1;
+    G.riskyPoints = 0;
+    if (G.indexWant != 1 && n > 0 && d > k && d <= 256 && !G.sortCells) {
+        const uint32_t m = (n + 99) / 100;
+        const size_t sb = (size_t)m * d * sizeof(T);
+        if (!reorder) {
+            GDSJ_TRY(G.aux.ensure(sb + 256));
+            GDSJ_TRY(G.hostSample.ensure(sb + 256));
+            const uint64_t tot = (uint64_t)m * d;
+            k_sample_gather<T><<<(uint32_t)((tot + 255) / 256), 256, 0, s>>>(data, n, d, m, G.aux.as<T>());
+            GDSJ_HIP(hipMemcpyAsync(G.hostSample.p, G.aux.p, sb, hipMemcpyDeviceToHost, s));
+            GDSJ_HIP(hipStreamSynchronize(s));
+        }
+        std::vector<double> mean(d, 0.0);
+        const T *hs = (const T *)G.hostSample.p;
+        for (uint32_t p = 0; p < m; p++)
+            for (uint32_t j = 0; j < d; j++) mean[j] += (double)hs[(size_t)p * d + j];
+        for (uint32_t j = 0; j < d; j++) mean[j] /= (double)m;
+        GDSJ_TRY(G.pcaBuf.ensure(((size_t)d * d + d + (size_t)MAXK * d) * 8 + 256));
+        double *dmean = G.pcaBuf.as<double>(), *dcov = dmean + d;
+        GDSJ_HIP(hipMemcpyAsync(dmean, mean.data(), (size_t)d * 8, hipMemcpyHostToDevice, s));
+        k_cov<T><<<dim3(d, d), 256, 0, s>>>(G.aux.as<T>(), m, d, dmean, dcov);
+        std::vector<double> C((size_t)d * d);
+        GDSJ_HIP(hipMemcpyAsync(C.data(), dcov, (size_t)d * d * 8, hipMemcpyDeviceToHost, s));
+        GDSJ_HIP(hipStreamSynchronize(s));
+        double capR = 0, capP = 0, total = 0;
+        pca_basis(C, d, k, G.perm, R, &capR, &capP, &total);
+        // auto: when the principal axes leave at most a twentieth of the
+        // variance that the reference's dims leave out (data near a rotated
+        // subspace of at most k dims)
+        pca = G.indexWant == 2 || (total - capP) < 0.05 * (total - capR);
+        if (pca) {
+            // |R|_2 <= sqrt(1 + |R R^T - I|_F)
+            double f = 0.0;
+            for (uint32_t a0 = 0; a0 < k; a0++)
+                for (uint32_t b0 = 0; b0 < k; b0++) {
+                    double dot = 0.0;
+                    for (uint32_t j = 0; j < d; j++) dot += R[(size_t)a0 * d + j] * R[(size_t)b0 * d + j];
+                    const double e = dot - (a0 == b0 ? 1.0 : 0.0);
+                    f += e * e;
+                }
+            delta = sqrt(1.0 + sqrt(f)) - 1.0 + 1e-12;
+        }
+    }
     KeyArgs a{};
     a.k = k;
     for (uint32_t j = 0; j < k; j++) a.pk[j] = G.perm[j];
@@ -459,6 +659,113 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         a.mult[j] = G.ga.mult[j];
         a.minArr[j] = G.minArr[j];
     }
+    const double uT = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
+    if (pca) {
+        // a point near a face of the reference's grid: the reference may
+        // miss a pair of it that the principal-axes grid finds
+        double qmax = 0.0;
+        for (uint32_t j = 0; j < k; j++) qmax = std::max(qmax, (double)G.ga.nCells[j]);
+        const double tau = ((double)d + (double)G.ilpTerms + 8.0) * uT + 4.1 * uT * (qmax + 1.0);
+        const uint32_t cap = 1u << 16;
+        GDSJ_TRY(G.scalars.ensure(256));
+        GDSJ_TRY(G.ybuf.ensure((size_t)cap * 8 + 256));
+        uint64_t *list = G.ybuf.as<uint64_t>();
+        uint32_t *cnt = G.scalars.as<uint32_t>() + 2;
+        GDSJ_HIP(hipMemsetAsync(cnt, 0, 4, s));
+        k_face_risk<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, a, (T)G.eps, tau, cap, cnt, list);
+        uint32_t nl = 0;
+        GDSJ_HIP(hipMemcpyAsync(&nl, cnt, 4, hipMemcpyDeviceToHost, s));
+        GDSJ_HIP(hipStreamSynchronize(s));
+        uint64_t risky = 0;
+        if (nl > cap) {
+            risky = nl;  // too many points on faces to check: the reference's dims
+        } else if (nl > 0) {
+            std::vector<uint64_t> hl(nl);
+            GDSJ_HIP(hipMemcpy(hl.data(), list, (size_t)nl * 8, hipMemcpyDeviceToHost));
+            std::sort(hl.begin(), hl.end());
+            const uint64_t belowBit = 1ull << 55;
+            for (uint64_t e : hl) {
+                if (!(e & belowBit)) continue;
+                // an entry above the face two cells up in the same dim
+                const uint64_t want = (e & ~belowBit) + 2u;
+                if (std::binary_search(hl.begin(), hl.end(), want)) risky++;
+            }
+        }
+        G.riskyPoints = risky;
+        if (risky) pca = false;
+    }
+    const T *keyData = data;  // the coordinates the cells are keyed on
+    uint32_t keyD = d;
+    double pcaPrune = 0.0;
+    if (pca) {
+        double *dR = G.pcaBuf.as<double>() + (size_t)d * d + d;
+        GDSJ_HIP(hipMemcpyAsync(dR, R.data(), (size_t)k * d * 8, hipMemcpyHostToDevice, s));
+        GDSJ_TRY(G.ybuf.ensure((size_t)n * k * sizeof(T) + 256));
+        T *y = G.ybuf.as<T>();
+        unsigned long long *nmax = G.scalars.as<unsigned long long>() + 8;
+        GDSJ_HIP(hipMemsetAsync(nmax, 0, 8, s));
+        k_project<T><<<ceil_div(n, 256), 256, (size_t)k * d * 8, s>>>(data, n, d, k, dR, y, nmax);
+        KeyArgs ay{};
+        ay.k = k;
+        for (uint32_t j = 0; j < k; j++) ay.pk[j] = j;
+        k_minmax<T><<<mmBlocks, 256, 0, s>>>(y, n, k, ay, part);
+        GDSJ_HIP(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(T), hipMemcpyDeviceToHost, s));
+        unsigned long long nbits = 0;
+        GDSJ_HIP(hipMemcpyAsync(&nbits, nmax, 8, hipMemcpyDeviceToHost, s));
+        GDSJ_HIP(hipStreamSynchronize(s));
+        T ymn[MAXK], ymx[MAXK];
+        double yabs = 0.0;
+        for (uint32_t j = 0; j < k; j++) {
+            ymn[j] = hp[j * 2];
+            ymx[j] = hp[j * 2 + 1];
+            for (uint32_t b = 1; b < mmBlocks; b++) {
+                T v0 = hp[((size_t)b * k + j) * 2], v1 = hp[((size_t)b * k + j) * 2 + 1];
+                ymn[j] = v0 < ymn[j] ? v0 : ymn[j];
+                ymx[j] = v1 > ymx[j] ? v1 : ymx[j];
+            }
+            yabs = std::max(yabs, std::max(fabs((double)ymn[j]), fabs((double)ymx[j])));
+        }
+        double n2;
+        memcpy(&n2, &nbits, 8);
+        const double xnorm = sqrt(n2) * (1.0 + 1e-12);
+        // projection error per coordinate: the double FMA sum (gamma_d of
+        // sum |R_cj x_j| <= |R_c| |x|) and the rounding to DTYPE
+        auto gam = [](double m, double u) { return m * u / (1.0 - m * u); };
+        const double err = gam((double)d + 1.0, 0x1p-53) * (1.0 + delta) * xnorm + uT * yabs * (1.0 + 1e-9);
+        // |x_a - x_b| of a pair the reference accepts (acc <= thr, acc within
+        // gamma of the sum of its rounded squared differences)
+        const double thr = sizeof(T) == 4 ? (double)gdsj_eps_threshold_f32((float)G.eps)
+                                          : gdsj_eps_threshold_f64(G.eps);
+        const double nt = (double)d + (double)G.ilpTerms + 2.0;
+        const double epsP = sqrt(thr / ((1.0 - gam(nt, uT)) * (1.0 - uT) * (1.0 - uT))) * (1.0 + 1e-12);
+        // per coordinate |y_a - y_b| <= w0; cells of width w >= w0 plus the
+        // rounding of the quotient (y - min) / w keep such pairs adjacent
+        const double w0 = (1.0 + delta) * epsP + 2.0 * err;
+        double span = 0.0;
+        for (uint32_t j = 0; j < k; j++) span = std::max(span, (double)ymx[j] - (double)ymn[j]);
+        const double qmax = span / w0 + 4.0;
+        if (!(4.1 * uT * qmax < 0.01)) {
+            set_error("principal-axes index: cells too fine for DTYPE");
+            return GDSJ_ERANGE;
+        }
+        double w = w0 / (1.0 - 4.1 * uT * qmax) * (1.0 + 4.0 * uT);
+        T wt = (T)w;
+        if ((double)wt < w) wt = sizeof(T) == 4 ? (T)nextafterf((float)wt, INFINITY) : (T)nextafter((double)wt, INFINITY);
+        G.cellW = (double)wt;
+        GDSJ_TRY(host_extent<T>(G, ymn, ymx));
+        a = ay;
+        for (uint32_t j = 0; j < k; j++) {
+            a.nCells[j] = G.ga.nCells[j];
+            a.mult[j] = G.ga.mult[j];
+            a.minArr[j] = G.minArr[j];
+        }
+        keyData = y;
+        keyD = k;
+        // the Euclidean |y_a - y_b| over the k axes of such a pair
+        const double w2 = (1.0 + delta) * epsP + 2.0 * sqrt((double)k) * err;
+        pcaPrune = w2 * w2 * (1.0 + 1e-9);
+        G.indexUsed = 2;
+    }
     // ---- keys + sort
     GDSJ_TRY(G.keyA.ensure((size_t)n * 8));
     GDSJ_TRY(G.keyB.ensure((size_t)n * 8));
@@ -484,7 +791,8 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         }
         k_cell_keys<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, a, (T)G.eps, kA, v0, outside, v0);
     } else {
-        k_cell_keys<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, a, (T)G.eps, kA, v0, outside, nullptr);
+        k_cell_keys<T><<<ceil_div(n, 256), 256, 0, s>>>(keyData, n, keyD, a, (T)G.cellW, kA, v0, outside,
+                                                         nullptr);
     }
     int alt = 0;
     int bits = bit_length(G.totalCells - 1);
@@ -526,6 +834,12 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     else
         k_soa<T><<<ceil_div(n, 256), 256, 0, s>>>(data, n, d, dperm, svals, G.xs.as<T>(), G.ld, idsOut, nonfinite);
     GDSJ_HIP(hipGetLastError());
+    if (pca) {
+        // the index coordinates in sorted order, for the run pruning
+        GDSJ_TRY(G.ixs.ensure((size_t)G.ld * k * sizeof(T) + 256));
+        k_gather_index<T><<<(uint32_t)((G.ld + 255) / 256), 256, 0, s>>>(G.ybuf.as<T>(), n, k, svals, G.ld,
+                                                                         G.ixs.as<T>());
+    }
     if (idsOut != G.ids.as<uint32_t>()) std::swap(G.ids, G.valB);
     uint32_t hdr[3] = {0, 0, 0};
     GDSJ_HIP(hipMemcpyAsync(&hdr[0], cidx + n, 4, hipMemcpyDeviceToHost, s));
@@ -569,10 +883,12 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     // pruning constants: a neighbor cell whose squared gap to every query of a
     // tile exceeds eps^2 (1 + 8 (D + 4) u) cannot hold a pair the DTYPE
     // predicate accepts (the accumulated sum is within (D + 3) u of the exact one)
+    // (principal axes: the squared Euclidean bound over the k axes of
+    // |y_a - y_b| for such a pair)
     for (uint32_t j = 0; j < k; j++) G.ga.minArr[j] = G.minArr[j];
-    G.ga.eps = G.eps;
+    G.ga.eps = G.cellW;
     G.ga.ulp = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
-    G.ga.prune_lim = G.eps * G.eps * (1.0 + 8.0 * (d + 4) * G.ga.ulp);
+    G.ga.prune_lim = pca ? pcaPrune : G.eps * G.eps * (1.0 + 8.0 * (d + 4) * G.ga.ulp);
     // ---- tiles
     GDSJ_TRY(G.cellAux.ensure(((size_t)nc + 1) * 4 * 2 + 256));
     uint32_t *tileCnt = G.cellAux.as<uint32_t>();

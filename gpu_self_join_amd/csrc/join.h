@@ -33,6 +33,9 @@ constexpr uint32_t SLOT_POS_LIMIT = 1u << 30;
 struct JoinArgs {
     const void *xs;
     uint64_t ld;
+    // the index coordinates in sorted order ([k][ld]): xs itself, or the
+    // principal-axes coordinates (Grid::ixs); the run pruning reads them
+    const void *ixs;
     // nullable: the row-major copy of xs (Grid::xa; up to 8 dims), from which
     // the kernels load candidate coordinates; xaStride elements per point
     const void *xa;

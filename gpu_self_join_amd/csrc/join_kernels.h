@@ -426,7 +426,7 @@ __device__ __forceinline__ void join_tile(const T *__restrict__ xs, const uint32
         nwin = a.ncombo;
         load_cell_coords(a.cellCoord, ga.ncell, cell, ga.k, cc);
         if (a.prune) {
-            tile_gaps<T>(xs, ld, q0, nq, cc, ga, glo, ghi);
+            tile_gaps<T>((const T *)a.ixs, ld, q0, nq, cc, ga, glo, ghi);
         } else {
             if (lane < MAXK) {
                 glo[lane] = 0.0;

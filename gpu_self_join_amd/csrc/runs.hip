@@ -188,7 +188,8 @@ template <typename T, int R>
 static void launch_cell_runs(Grid &G, uint32_t blocks, uint32_t ncombo, uint32_t na, uint32_t nb, uint32_t split,
                              bool prune, int stamp, uint32_t *maskW, hipStream_t s) {
     const size_t lds = (size_t)(256 / GDSJ_WAVE) * (na + nb) * sizeof(double);
-    k_cell_runs<T, R><<<blocks, 256, lds, s>>>(G.xs.as<T>(), G.ld, G.ga, G.cellStart.as<uint32_t>(),
+    const T *ixs = G.indexUsed == 2 ? G.ixs.as<T>() : G.xs.as<T>();  // the index coordinates
+    k_cell_runs<T, R><<<blocks, 256, lds, s>>>(ixs, G.ld, G.ga, G.cellStart.as<uint32_t>(),
                                                G.cellCoord.as<uint32_t>(), G.cellKey.as<uint64_t>(),
                                                G.hkeys.as<uint64_t>(), G.hvals.as<uint32_t>(), G.comboTab.as<uint4>(),
                                                ncombo, na, nb, split, G.runStride, prune ? 1 : 0, stamp,

@@ -120,6 +120,16 @@ typedef struct gdsj_params {
                                evaluated in the reference's order, so the counts are unchanged
                                (speed only).  0 auto (on), 1 on, 2 every pair in the band (the
                                reference's arithmetic for all) */
+    uint32_t index;         /* the dims the grid indexes (speed only: the pair test is unchanged).
+                               1 the reference's k dims of largest variance (REORDER); 2 the top
+                               k principal axes of the REORDER sample (y = R x, cells slightly
+                               wider than eps so that every pair the reference's arithmetic
+                               accepts lies in adjacent cells), used only when no point lies
+                               within rounding of a face of the reference's own grid (so that
+                               the reference finds every such pair too), with more dims than k,
+                               up to 256 dims, no unicomp and no SORT; else 1.  0 auto: 2 where
+                               the principal axes leave at most a twentieth of the variance the
+                               reference's dims leave (data near a rotated subspace) */
 } gdsj_params;
 
 /* Timing and diagnostics of the last join on a handle. */
@@ -151,6 +161,9 @@ typedef struct gdsj_stats {
     double ms_total;             /* sum of the phases above except training */
     uint32_t perm[64];           /* dimension order after REORDER (first 64) */
     uint64_t mfma_tiles;         /* tiles whose pair tests ran on the matrix cores (gdsj_params.mfma) */
+    uint64_t index_used;         /* 1 the reference's dims, 2 the principal axes (gdsj_params.index) */
+    uint64_t index_risky;        /* principal axes asked for: points within rounding of a face of the
+                                    reference's grid (any: the reference's dims were used) */
 } gdsj_stats;
 
 typedef struct gdsj_handle gdsj_handle;

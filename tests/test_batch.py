@@ -195,7 +195,7 @@ def test_batched_workload_equals_single_joins(gpu):
         print("batched stats", bj.stats(), flush=True)
         for i in range(m):
             c, off, ids = r.instance(i)
-            s = sj.join(xs[i], eps[i], filter=_lib.FILTER_PLAIN)
+            s = sj.join(xs[i], eps[i], filter=_lib.FILTER_PLAIN, index=1)  # the batch's index
             assert torch.equal(c, s.counts), i
             assert torch.equal(ids, s.ids), i
             x = host[i]
