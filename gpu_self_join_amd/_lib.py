@@ -51,7 +51,8 @@ class GdsjParams(C.Structure):
                 ("hitmask", C.c_uint32), ("ilp_fold", C.c_uint32),
                 ("vfilter", C.c_uint32), ("sort", C.c_uint32),
                 ("cpu_sum", C.c_uint32), ("mfma", C.c_uint32),
-                ("ffilter", C.c_uint32), ("index", C.c_uint32)]
+                ("ffilter", C.c_uint32), ("index", C.c_uint32),
+                ("index_dims", C.c_uint32)]
 
 
 class GdsjStats(C.Structure):

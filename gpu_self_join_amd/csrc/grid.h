@@ -85,6 +85,7 @@ struct Grid {
     // coordinates in sorted order ([k][ld], principal axes only), which the
     // run pruning reads instead of xs; ybuf: the same in input order.
     int indexWant = 1;
+    uint32_t indexDims = 0;  // principal axes indexed (0: k); the grid's k becomes it
     uint32_t ilpTerms = 0;
     int indexUsed = 1;
     uint64_t riskyPoints = 0;

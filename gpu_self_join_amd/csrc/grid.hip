@@ -259,13 +259,51 @@ __global__ void k_gather_index(const T *__restrict__ y, uint32_t n, uint32_t k, 
         ixs[(uint64_t)c * ld + p] = p < n ? y[(uint64_t)order[p] * k + c] : (T)__builtin_nan("");
 }
 
-// The top-k principal directions of the covariance C (d x d) by subspace
+// Eigen-decomposition of the symmetric k x k matrix A (row-major) by cyclic
+// Jacobi rotations: eigenvalues on return in A's diagonal, eigenvectors in
+// the columns of V.
+static void jacobi_sym(std::vector<double> &A, std::vector<double> &V, uint32_t k) {
+    V.assign((size_t)k * k, 0.0);
+    for (uint32_t i = 0; i < k; i++) V[(size_t)i * k + i] = 1.0;
+    for (int sweep = 0; sweep < 30; sweep++) {
+        double off = 0.0;
+        for (uint32_t i = 0; i < k; i++)
+            for (uint32_t j = i + 1; j < k; j++) off += A[(size_t)i * k + j] * A[(size_t)i * k + j];
+        if (off < 1e-30) break;
+        for (uint32_t p = 0; p < k; p++)
+            for (uint32_t q = p + 1; q < k; q++) {
+                const double apq = A[(size_t)p * k + q];
+                if (apq == 0.0) continue;
+                const double th = (A[(size_t)q * k + q] - A[(size_t)p * k + p]) / (2.0 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+                for (uint32_t r = 0; r < k; r++) {  // A <- A J
+                    const double arp = A[(size_t)r * k + p], arq = A[(size_t)r * k + q];
+                    A[(size_t)r * k + p] = c * arp - sn * arq;
+                    A[(size_t)r * k + q] = sn * arp + c * arq;
+                }
+                for (uint32_t r = 0; r < k; r++) {  // A <- J^T A
+                    const double apr = A[(size_t)p * k + r], aqr = A[(size_t)q * k + r];
+                    A[(size_t)p * k + r] = c * apr - sn * aqr;
+                    A[(size_t)q * k + r] = sn * apr + c * aqr;
+                }
+                for (uint32_t r = 0; r < k; r++) {  // V <- V J
+                    const double vrp = V[(size_t)r * k + p], vrq = V[(size_t)r * k + q];
+                    V[(size_t)r * k + p] = c * vrp - sn * vrq;
+                    V[(size_t)r * k + q] = sn * vrp + c * vrq;
+                }
+            }
+    }
+}
+
+// The top-k principal directions of the covariance C (d x d): subspace
 // iteration from the unit vectors of the k dims of largest variance
-// (perm[0..k)), modified Gram-Schmidt after every product.  R (k x d, rows
-// orthonormal up to rounding); *capR / *capP: variance captured by those k
-// dims / by R; *total: trace of C.
-static void pca_basis(const std::vector<double> &C, uint32_t d, uint32_t k, const uint32_t *perm,
-                      std::vector<double> &R, double *capR, double *capP, double *total) {
+// (perm[0..k)), modified Gram-Schmidt after every product, then the
+// Rayleigh-Ritz rotation that orders them by variance.  R: the first kp of
+// them (kp x d, rows orthonormal up to rounding); *capK: variance captured
+// by all k; *total: trace of C.
+static void pca_basis(const std::vector<double> &C, uint32_t d, uint32_t k, uint32_t kp, const uint32_t *perm,
+                      std::vector<double> &R, double *capK, double *total) {
     std::vector<double> Q((size_t)d * k, 0.0), Z((size_t)d * k);
     for (uint32_t c = 0; c < k; c++) Q[(size_t)perm[c] * k + c] = 1.0;
     auto orth = [&](std::vector<double> &M) {
@@ -282,32 +320,50 @@ static void pca_basis(const std::vector<double> &C, uint32_t d, uint32_t k, cons
             if (nn == 0.0) M[(size_t)perm[c] * k + c] = 1.0;  // degenerate: keep the unit vector
         }
     };
-    for (int it = 0; it < 12; it++) {
+    auto mulC = [&](const std::vector<double> &In, std::vector<double> &Out) {
         for (uint32_t i = 0; i < d; i++)
             for (uint32_t c = 0; c < k; c++) {
                 double acc = 0.0;
-                for (uint32_t j = 0; j < d; j++) acc += C[(size_t)i * d + j] * Q[(size_t)j * k + c];
-                Z[(size_t)i * k + c] = acc;
+                for (uint32_t j = 0; j < d; j++) acc += C[(size_t)i * d + j] * In[(size_t)j * k + c];
+                Out[(size_t)i * k + c] = acc;
             }
+    };
+    for (int it = 0; it < 12; it++) {
+        mulC(Q, Z);
         orth(Z);
         Q.swap(Z);
     }
-    R.assign((size_t)k * d, 0.0);
-    *capP = 0.0;
-    for (uint32_t c = 0; c < k; c++) {
-        for (uint32_t j = 0; j < d; j++) R[(size_t)c * d + j] = Q[(size_t)j * k + c];
-        double q = 0.0;
-        for (uint32_t i = 0; i < d; i++) {
-            double row = 0.0;
-            for (uint32_t j = 0; j < d; j++) row += C[(size_t)i * d + j] * Q[(size_t)j * k + c];
-            q += Q[(size_t)i * k + c] * row;
+    // Rayleigh-Ritz: A = Q^T C Q, A = V diag V^T, Q <- Q V (by variance)
+    mulC(Q, Z);
+    std::vector<double> A((size_t)k * k), V;
+    for (uint32_t a0 = 0; a0 < k; a0++)
+        for (uint32_t b0 = 0; b0 < k; b0++) {
+            double acc = 0.0;
+            for (uint32_t j = 0; j < d; j++) acc += Q[(size_t)j * k + a0] * Z[(size_t)j * k + b0];
+            A[(size_t)a0 * k + b0] = 0.5 * acc;
         }
-        *capP += q;
-    }
+    for (uint32_t a0 = 0; a0 < k; a0++)  // symmetrise
+        for (uint32_t b0 = a0 + 1; b0 < k; b0++) {
+            const double m = A[(size_t)a0 * k + b0] + A[(size_t)b0 * k + a0];
+            A[(size_t)a0 * k + b0] = A[(size_t)b0 * k + a0] = m;
+        }
+    for (uint32_t a0 = 0; a0 < k; a0++) A[(size_t)a0 * k + a0] *= 2.0;
+    jacobi_sym(A, V, k);
+    std::vector<uint32_t> order(k);
+    for (uint32_t c = 0; c < k; c++) order[c] = c;
+    std::sort(order.begin(), order.end(),
+              [&](uint32_t x, uint32_t y) { return A[(size_t)x * k + x] > A[(size_t)y * k + y]; });
+    *capK = 0.0;
+    for (uint32_t c = 0; c < k; c++) *capK += A[(size_t)c * k + c];
+    R.assign((size_t)kp * d, 0.0);
+    for (uint32_t c = 0; c < kp; c++)
+        for (uint32_t j = 0; j < d; j++) {
+            double acc = 0.0;
+            for (uint32_t b = 0; b < k; b++) acc += Q[(size_t)j * k + b] * V[(size_t)b * k + order[c]];
+            R[(size_t)c * d + j] = acc;
+        }
     *total = 0.0;
     for (uint32_t j = 0; j < d; j++) *total += C[(size_t)j * d + j];
-    *capR = 0.0;
-    for (uint32_t c = 0; c < k; c++) *capR += C[(size_t)perm[c] * d + perm[c]];
 }
 
 // SORT: the cell of every sorted position (inclusive count of cell heads - 1)
@@ -544,7 +600,8 @@ static int host_extent(Grid &G, const T *mn, const T *mx) {
 
 template <typename T>
 static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int order_tiles) {
-    const uint32_t n = G.n, d = G.d, k = G.k;
+    const uint32_t n = G.n, d = G.d;
+    uint32_t k = G.k;  // the principal-axes grid may index fewer dims (kp)
     G.ga.k = k;
     // ---- REORDER
     for (uint32_t l = 0; l < d; l++) G.perm[l] = l;
@@ -581,6 +638,7 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     G.cellW = G.eps;
     G.indexUsed = 1;
     G.riskyPoints = 0;
+    const uint32_t kp = G.indexDims && G.indexDims < k ? G.indexDims : k;  // principal axes indexed
     if (G.indexWant != 1 && n > 0 && d > k && d <= 256 && !G.sortCells) {
         const uint32_t m = (n + 99) / 100;
         const size_t sb = (size_t)m * d * sizeof(T);
@@ -604,17 +662,18 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         std::vector<double> C((size_t)d * d);
         GDSJ_HIP(hipMemcpyAsync(C.data(), dcov, (size_t)d * d * 8, hipMemcpyDeviceToHost, s));
         GDSJ_HIP(hipStreamSynchronize(s));
-        double capR = 0, capP = 0, total = 0;
-        pca_basis(C, d, k, G.perm, R, &capR, &capP, &total);
-        // auto: when the principal axes leave at most a twentieth of the
-        // variance that the reference's dims leave out (data near a rotated
-        // subspace of at most k dims)
+        double capP = 0, total = 0, capR = 0;
+        pca_basis(C, d, k, kp, G.perm, R, &capP, &total);
+        for (uint32_t c = 0; c < k; c++) capR += C[(size_t)G.perm[c] * d + G.perm[c]];  // the reference's dims
+        // auto: when k principal axes leave at most a twentieth of the
+        // variance that the reference's k dims leave out (data near a
+        // rotated subspace of at most k dims)
         pca = G.indexWant == 2 || (total - capP) < 0.05 * (total - capR);
         if (pca) {
             // |R|_2 <= sqrt(1 + |R R^T - I|_F)
             double f = 0.0;
-            for (uint32_t a0 = 0; a0 < k; a0++)
-                for (uint32_t b0 = 0; b0 < k; b0++) {
+            for (uint32_t a0 = 0; a0 < kp; a0++)
+                for (uint32_t b0 = 0; b0 < kp; b0++) {
                     double dot = 0.0;
                     for (uint32_t j = 0; j < d; j++) dot += R[(size_t)a0 * d + j] * R[(size_t)b0 * d + j];
                     const double e = dot - (a0 == b0 ? 1.0 : 0.0);
@@ -698,6 +757,9 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     uint32_t keyD = d;
     double pcaPrune = 0.0;
     if (pca) {
+        k = kp;
+        G.k = kp;
+        G.ga.k = kp;
         double *dR = G.pcaBuf.as<double>() + (size_t)d * d + d;
         GDSJ_HIP(hipMemcpyAsync(dR, R.data(), (size_t)k * d * 8, hipMemcpyHostToDevice, s));
         GDSJ_TRY(G.ybuf.ensure((size_t)n * k * sizeof(T) + 256));
