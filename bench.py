@@ -396,9 +396,11 @@ def bench_workload(name, args, variants, steps, warmup, cpu_budget, cpu_frac, ex
         if vname == "nostamp":
             kw["stamp"] = 0  # the full neighborhood (STAMP only changes which side evaluates a pair)
         if vname == "plain":
-            # the plain grid join without the MFMA filter (the reference's
-            # method on this hardware): few steps, it is the slow reference point
+            # the plain grid join without the MFMA filter on the reference's
+            # dims (the reference's method on this hardware): few steps, it is
+            # the slow reference point
             kw["filter"] = 0
+            kw["index"] = 1
             st_, wu = min(steps, 2 if d < 64 else 1), 0
         total, ms, per = run_variant(sj, pts, eps, kw, st_, wu, total)
         sj.close()

@@ -582,12 +582,12 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     // the index: the reference's dims, or the principal axes (not with
     // unicomp, whose ownership rules are stated on the reference's cells)
     h->G.indexWant = h->stamp ? 1 : (h->p.index == 0 ? 0 : (int)h->p.index);
-    // principal axes indexed: with the Gram filter (whose tiles share each
-    // candidate operand among their queries) two fewer than k from k = 6,
-    // for cells that hold more queries (measured on 64-D data near an 8-D
-    // subspace, k = 8: 5 to 7 axes 22.4 ms per join, 8 axes 32.4 ms)
-    h->G.indexDims = h->p.index_dims ? h->p.index_dims
-                                     : (h->use_gram && h->p.k >= 6 ? h->p.k - 2 : h->p.k);
+    // principal axes indexed: two fewer than k from k = 6, for cells that
+    // hold more queries sharing each candidate (measured with the Gram filter
+    // on 64-D data near an 8-D subspace, k = 8: 5 to 7 axes 22.4 ms per
+    // join, 8 axes 32.4 ms); the same for every filter, so that their rows
+    // come in the same order
+    h->G.indexDims = h->p.index_dims ? h->p.index_dims : (h->p.k >= 6 ? h->p.k - 2 : h->p.k);
     h->G.ilpTerms = h->p.ilp;
     // the row-major copy for the plain kernels' candidate loads (up to 16 dims)
     h->G.aosWant = !h->use_gram && dims <= 16 ? 1 : 0;

@@ -131,8 +131,8 @@ typedef struct gdsj_params {
                                the principal axes leave at most a twentieth of the variance the
                                reference's dims leave (data near a rotated subspace) */
     uint32_t index_dims;    /* principal axes the grid indexes when index 2 is used (at most k;
-                               0: k).  Fewer axes give larger cells: more candidates per query,
-                               but more queries per tile sharing them */
+                               0: k - 2 from k = 6, else k).  Fewer axes give larger cells: more
+                               candidates per query, but more queries per tile sharing them */
 } gdsj_params;
 
 /* Timing and diagnostics of the last join on a handle. */
