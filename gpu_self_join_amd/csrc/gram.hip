@@ -150,7 +150,10 @@ __global__ void k_gram_convert(const T *__restrict__ xs, uint64_t ld, uint32_t n
     }
     const double e = sqrt(E2) * (1.0 + 0x1p-30) + sqrt(X2) * 0x1p-50;
     const double k = sigma * eps_p * 0.5 + e;
-    const double w = 0.5 * (N * (1.0 - 0x1p-40) - cfac * N1 - L) - (1.0 + 0x1p-20) * k * k;
+    // cfac covers the MFMA's FP32 accumulation of the products (and the
+    // initial -W_q it accumulates onto, |W_q| <= N1_q / 2 + k_q^2: cfac
+    // holds twice the term count's roundings, and the k^2 factor the same)
+    const double w = 0.5 * (N * (1.0 - 0x1p-40) - cfac * N1 - L) - (1.0 + 0x1p-20 + cfac) * k * k;
     W[s] = __double2float_rd(w);
     N2[s] = N;
     if (s == 0) sigOut[0] = sigma;
@@ -209,7 +212,8 @@ static int prepare_t(GramData &gd, const Grid &G, hipStream_t s) {
     const double eps_p = G.eps * (1.0 + 2.0 * (d + 5) * uT);
     // MFMA FP32 accumulation of (1 or 3) x dpad exact products, with margin
     // for the FP32 rounding of W_q + W_p and of the stored N
-    const double cfac = (2.0 * (gd.split ? 3.0 : 1.0) * dpad + 16.0) * 0x1p-24;
+    // (and, as the kernel accumulates onto -W_q, once more for that term)
+    const double cfac = (4.0 * (gd.split ? 3.0 : 1.0) * dpad + 24.0) * 0x1p-24;
     k_gram_convert<T, LP><<<ceil_div(n, 256), 256, 0, s>>>(
         xs, G.ld, n, d, dpad, mean, maxdev, eps_p, cfac, gd.split, (typename Lp<LP>::scalar *)gd.h.p,
         gd.split ? (typename Lp<LP>::scalar *)gd.l.p : nullptr, gd.w.as<float>(), gd.n2.as<double>(),

@@ -123,9 +123,10 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
     }
     __builtin_amdgcn_wave_barrier();
 
-    // query weights (INFINITY past nq) in LDS: row r at s_wq[w][r]; the
-    // rows a lane's accumulator entries 4j..4j+3 cover are consecutive
-    for (uint32_t r = lane; r < TQ; r += GDSJ_WAVE) s_wq[w][r] = r < nq ? W[q0 + r] : INFINITY;
+    // negated query weights (-INFINITY past nq) in LDS: row r at s_wq[w][r];
+    // the rows a lane's accumulator entries 4j..4j+3 cover are consecutive,
+    // so they load as the MFMAs' initial accumulator: S - W_q
+    for (uint32_t r = lane; r < TQ; r += GDSJ_WAVE) s_wq[w][r] = r < nq ? -W[q0 + r] : -INFINITY;
     __builtin_amdgcn_wave_barrier();
     vec A[RB][KS];
     vec Al[RB][KS];
@@ -303,12 +304,19 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                 if (SPLIT) Bl[ks] = Bln[ks];
             }
             if (cb + 32 < whi) load_block(cb + 32);
-            // per row block: the MFMAs (those of the next row block are issued
-            // before this one's compares), then one wave mask per
-            // accumulator entry: bit l of the mask = the pair (row of the
-            // entry in lane l, column l & 31) survives the filter
-            auto gram_rb = [&](int rb) {
-                f32x16 acc = {};
+            // per row block: the MFMAs, accumulating onto -W_q, then one wave
+            // mask per accumulator entry: bit l of the mask = the pair (row of
+            // the entry in lane l, column l & 31) survives the filter,
+            // S - W_q >= W_p
+            auto gram_rb = [&](int rb, const float4 (&wv)[4]) {
+                f32x16 acc;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    acc[4 * j + 0] = wv[j].x;
+                    acc[4 * j + 1] = wv[j].y;
+                    acc[4 * j + 2] = wv[j].z;
+                    acc[4 * j + 3] = wv[j].w;
+                }
 #pragma unroll
                 for (int ks = 0; ks < KS; ks++) {
                     acc = mfma32<LP>(A[rb][ks], B[ks], acc);
@@ -326,17 +334,17 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
                 if (rb >= (int)nrb) break;
-                const f32x16 acc = gram_rb(rb);
+                float4 wv[4];  // -W_q of the rows of this lane's entries
+#pragma unroll
+                for (int j = 0; j < 4; j++) wv[j] = *reinterpret_cast<const float4 *>(&s_wq[w][rb * 32 + j * 8 + half * 4]);
+                const f32x16 acc = gram_rb(rb, wv);
                 // the common case, no survivor in the row block, costs a
                 // compare per entry and one branch
-                float4 wv[4];
                 uint64_t any = 0;
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    wv[j] = *reinterpret_cast<const float4 *>(&s_wq[w][rb * 32 + j * 8 + half * 4]);
-                    any |= ballot(acc[4 * j + 0] >= wv[j].x + wc) | ballot(acc[4 * j + 1] >= wv[j].y + wc) |
-                           ballot(acc[4 * j + 2] >= wv[j].z + wc) | ballot(acc[4 * j + 3] >= wv[j].w + wc);
-                }
+                for (int j = 0; j < 4; j++)
+                    any |= ballot(acc[4 * j + 0] >= wc) | ballot(acc[4 * j + 1] >= wc) | ballot(acc[4 * j + 2] >= wc) |
+                           ballot(acc[4 * j + 3] >= wc);
                 if (any == 0) continue;
                 if (qn + 32 * 32 > QCAP) flush(qn & ~(GDSJ_WAVE - 1u));
 #pragma unroll
@@ -344,8 +352,7 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
                         const int i = 4 * j + e;
-                        const float wq = e == 0 ? wv[j].x : (e == 1 ? wv[j].y : (e == 2 ? wv[j].z : wv[j].w));
-                        const bool keep = acc[i] >= wq + wc;
+                        const bool keep = acc[i] >= wc;
                         const uint64_t m = ballot(keep);
                         if (m == 0) continue;
                         const uint32_t r = rb * 32 + j * 8 + half * 4 + e;
@@ -360,7 +367,8 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                             qr[at] = (uint8_t)(hb + (uint32_t)(rb * 32 + j * 8 + e));
                             if (ERR) {
                                 const double s2 = aux.sig[0] * aux.sig[0];
-                                const double est = aux.nrm[q0 + r] + aux.nrm[sp] - 2.0 * (double)acc[i];
+                                const float nwq = e == 0 ? wv[j].x : (e == 1 ? wv[j].y : (e == 2 ? wv[j].z : wv[j].w));
+                                const double est = aux.nrm[q0 + r] + aux.nrm[sp] - 2.0 * ((double)acc[i] - (double)nwq);
                                 double ex = 0.0;
                                 for (uint32_t l = 0; l < a.d; l++) {
                                     const double t = (double)xs[(uint64_t)l * ld + sp] -
