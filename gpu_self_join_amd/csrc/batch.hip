@@ -41,10 +41,15 @@ __global__ void k_batch_reset(uint32_t *ctr) {
     if (threadIdx.x < BQ_N) ctr[threadIdx.x] = 0u;
 }
 
+// items in a fixed wave-strided order: a tile's compaction is short, so a
+// queue atomic per item would set the pace
 __global__ __launch_bounds__(256) void k_batch_compact(const JoinArgs *__restrict__ inst, const BatchLaunch L) {
     const uint32_t nitems = batch_items(L);
-    uint32_t i, tile;
-    while (batch_next(L, nitems, i, tile)) {
+    const uint32_t nw = gridDim.x * (blockDim.x / GDSJ_WAVE);
+    for (uint32_t it = uniform(blockIdx.x * (blockDim.x / GDSJ_WAVE) + threadIdx.x / GDSJ_WAVE); it < nitems;
+         it += nw) {
+        uint32_t i, tile;
+        batch_item(L, it, i, tile);
         const JoinArgs &a = inst_args(inst, i);
         compact_tile(a, tile);
     }
@@ -283,9 +288,12 @@ int enqueue_run(gdsj_batch *b, uint32_t *counts, uint64_t *offsets, bool slotted
     } else {
         GDSJ_TRY(pass(b, launch_of(b, BQ_SLOT, lpt), PASS_SLOT, s, &blocks));
         GDSJ_TRY(exclusive_scan_u32_u64(counts, offsets, b->npts, true, b->scanTmp.p, s));
+        // the compaction is latency-bound (a few dependent loads per tile):
+        // its own grid of 4-wave blocks, 8 per CU, whatever the join passes'
+        // tuned shape (one wave per block at 3 per SIMD held it to 3 waves
+        // per CU)
         BatchLaunch C = launch_of(b, BQ_COMPACT, lpt);
-        const uint32_t threads = b->tune.waves_per_block * GDSJ_WAVE;
-        k_batch_compact<<<blocks ? blocks : 1024, threads, 0, s>>>(b->dArgs.as<JoinArgs>(), C);
+        k_batch_compact<<<2048, 256, 0, s>>>(b->dArgs.as<JoinArgs>(), C);
         GDSJ_HIP(hipGetLastError());
         BatchLaunch O = launch_of(b, BQ_OVF, true);
         O.order = b->slotOvf.as<uint32_t>();

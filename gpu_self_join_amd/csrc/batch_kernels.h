@@ -28,6 +28,18 @@ struct BatchLaunch {
 // Takes the next item: false once the queue is exhausted.  The instance is
 // the last i with tileBase[i] <= g (empty instances share their successor's
 // base and are skipped by the search).
+// the (instance, tile) of queue item it
+__device__ __forceinline__ void batch_item(const BatchLaunch &L, uint32_t it, uint32_t &inst, uint32_t &tile) {
+    const uint32_t g = L.order ? uniform(L.order[it]) : it;
+    uint32_t lo = 0, hi = L.ninst;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (uniform(L.tileBase[mid]) <= g) lo = mid; else hi = mid;
+    }
+    inst = lo;
+    tile = g - uniform(L.tileBase[lo]);
+}
+
 __device__ __forceinline__ bool batch_next(const BatchLaunch &L, uint32_t nitems, uint32_t &inst,
                                            uint32_t &tile) {
     uint32_t it = 0;
