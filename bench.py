@@ -477,7 +477,8 @@ def bench_workload(name, args, variants, steps, warmup, cpu_budget, cpu_frac, ex
         "config": {"workload": name, "n": n, "dims": d, "eps": eps, "total_pairs": total,
                    "pairs_per_point": total / n, **base, "estimator": best},
         "variants": variants_out,
-        "grid": {"nonempty_cells": s["n_nonempty_cells"], "tiles": s["n_tiles"], "candidates": s["candidates"]},
+        "grid": {"nonempty_cells": s["n_nonempty_cells"], "tiles": s["n_tiles"], "candidates": s["candidates"],
+                 "index": "principal axes" if s.get("index_used") == 2 else "reference dims"},
         "speedup_vs_plain": (results["plain"]["ms_per_step"] / r["ms_per_step"]) if "plain" in results else None,
         # the join passes alone (grid resident): counting or slotted pass,
         # scan, compaction and fill (the grid build is in ms_per_step)
