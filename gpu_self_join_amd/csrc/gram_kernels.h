@@ -338,14 +338,14 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
 #pragma unroll
                 for (int j = 0; j < 4; j++) wv[j] = *reinterpret_cast<const float4 *>(&s_wq[w][rb * 32 + j * 8 + half * 4]);
                 const f32x16 acc = gram_rb(rb, wv);
-                // the common case, no survivor in the row block, costs a
-                // compare per entry and one branch
-                uint64_t any = 0;
+                // the common case, no survivor in the row block, costs the
+                // lane's largest entry (three-input max steps), one compare
+                // and one branch (entries are finite or -inf)
+                float mx = acc[0];
 #pragma unroll
-                for (int j = 0; j < 4; j++)
-                    any |= ballot(acc[4 * j + 0] >= wc) | ballot(acc[4 * j + 1] >= wc) | ballot(acc[4 * j + 2] >= wc) |
-                           ballot(acc[4 * j + 3] >= wc);
-                if (any == 0) continue;
+                for (int i = 1; i + 1 < 16; i += 2) mx = fmaxf(fmaxf(mx, acc[i]), acc[i + 1]);
+                mx = fmaxf(mx, acc[15]);
+                if (ballot(mx >= wc) == 0) continue;
                 if (qn + 32 * 32 > QCAP) flush(qn & ~(GDSJ_WAVE - 1u));
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
