@@ -52,7 +52,7 @@ class GdsjParams(C.Structure):
                 ("vfilter", C.c_uint32), ("sort", C.c_uint32),
                 ("cpu_sum", C.c_uint32), ("mfma", C.c_uint32),
                 ("ffilter", C.c_uint32), ("index", C.c_uint32),
-                ("index_dims", C.c_uint32)]
+                ("index_dims", C.c_uint32), ("gram_axes", C.c_uint32)]
 
 
 class GdsjStats(C.Structure):
@@ -65,7 +65,8 @@ class GdsjStats(C.Structure):
                 ("ms_scan", C.c_double), ("ms_compact", C.c_double), ("ms_fill", C.c_double),
                 ("ms_sort", C.c_double), ("ms_train", C.c_double), ("ms_total", C.c_double),
                 ("perm", C.c_uint32 * 64), ("mfma_tiles", C.c_uint64),
-                ("index_used", C.c_uint64), ("index_risky", C.c_uint64)]
+                ("index_used", C.c_uint64), ("index_risky", C.c_uint64),
+                ("gram_axes", C.c_uint64)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "perm"}

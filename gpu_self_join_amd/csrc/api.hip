@@ -259,6 +259,10 @@ static int resolve_params(const gdsj_params *in, uint32_t dims, gdsj_params *out
         set_error("ffilter=%u: 0 auto, 1 on, 2 every pair by the reference's arithmetic", out->ffilter);
         return GDSJ_EINVAL;
     }
+    if (out->gram_axes != 0 && out->gram_axes != 1 && out->gram_axes != 16 && out->gram_axes != 32) {
+        set_error("gram_axes=%u: 0 auto, 1 the points' dims, 16 or 32 principal axes", out->gram_axes);
+        return GDSJ_EINVAL;
+    }
     if (out->ilp_fold > 1) {
         set_error("ilp_fold=%u: 0 or 1", out->ilp_fold);
         return GDSJ_EINVAL;
@@ -589,6 +593,8 @@ int gdsj_join_device(gdsj_handle *h, const void *d_data, uint32_t n, uint32_t di
     // come in the same order
     h->G.indexDims = h->p.index_dims ? h->p.index_dims : (h->p.k >= 6 ? h->p.k - 2 : h->p.k);
     h->G.ilpTerms = h->p.ilp;
+    // the Gram filter's operands on principal axes (its basis comes with the index's)
+    h->G.gramAxesWant = h->use_gram ? (int)h->p.gram_axes : 1;
     // the row-major copy for the plain kernels' candidate loads (up to 16 dims)
     h->G.aosWant = !h->use_gram && dims <= 16 ? 1 : 0;
     GDSJ_TRY(grid_build(h->G, d_data, n, dims, dtype, epsilon, h->p.k, h->p.reorder, tq, h->p.queryreorder, s));
@@ -719,6 +725,7 @@ static int join_passes(gdsj_handle *h, uint32_t n, uint32_t dims, uint32_t *d_co
     h->st.total_cells = h->G.totalCells;
     h->st.index_used = (uint64_t)h->G.indexUsed;
     h->st.index_risky = h->G.riskyPoints;
+    h->st.gram_axes = h->use_gram ? h->gram.axes : 0u;
     h->st.n_tiles = h->G.ntiles;
     h->st.n_runs = h->G.haveRuns ? 0 : (uint64_t)h->G.ntiles * a.ncombo;
     h->st.est_overflow_tiles = h->novf;

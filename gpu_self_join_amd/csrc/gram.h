@@ -12,6 +12,11 @@
 // so only pairs with S >= W_q + W_p are rechecked by direct differencing in
 // DTYPE with the reference's accumulation order (kernel.cu:307-365).  The
 // filter never decides a pair: the result equals the plain join's.
+// For data near a low-dimensional subspace h may instead hold the point's
+// coordinates on m principal axes, y = P (x - o): |P v| <= |P|_2 |v| for
+// every v, so with eps' widened by |P|_2 and e by the rounding of the
+// projection the same test keeps every pair within eps' (a filter on m
+// instead of d dims: m / d of the operand bytes and MFMAs).
 #pragma once
 #include "common.h"
 #include "grid.h"
@@ -26,6 +31,8 @@ struct GramData {
     DevBuf n2;       // [n] double: N = |h + l|^2 (filter-error measurement)
     DevBuf scratch;  // per-dim means [d], max deviations [d], sigma
     DevBuf xr;       // [n][xrs] DTYPE: the sorted points row-major (the exact rechecks' loads)
+    DevBuf pm;       // [axes][d] double: the principal axes h is taken on (axes > 0)
+    uint32_t axes = 0;  // 0: h holds the points' own dims; else y = P (x - o) on this many axes
     uint32_t xrs = 0;  // row stride: dims rounded up to a multiple of 16
     uint32_t dpad = 0;
     int lowp = 0;    // 0 bf16, 1 fp16

@@ -22,6 +22,8 @@ void GramData::release() {
     h.release();
     l.release();
     xr.release();
+    pm.release();
+    axes = 0;
     xrs = 0;
     w.release();
     n2.release();
@@ -99,6 +101,16 @@ __device__ __forceinline__ double lp_round(double v, typename Lp<LP>::scalar &ou
     return hd;
 }
 
+// W from the point's N, N1, L, e and sigma eps' / 2.  cfac covers the MFMA's
+// FP32 accumulation of the products (and the initial -W_q it accumulates
+// onto, |W_q| <= N1_q / 2 + k_q^2: cfac holds twice the term count's
+// roundings, and the k^2 factor the same).
+__device__ __forceinline__ float gram_weight(double N, double N1, double L, double e, double half_eps, double cfac) {
+    const double k = half_eps + e;
+    const double w = 0.5 * (N * (1.0 - 0x1p-40) - cfac * N1 - L) - (1.0 + 0x1p-20 + cfac) * k * k;
+    return __double2float_rd(w);
+}
+
 template <typename T, int LP>
 __global__ void k_gram_convert(const T *__restrict__ xs, uint64_t ld, uint32_t n, uint32_t d, uint32_t dpad,
                                const double *__restrict__ mean, const double *__restrict__ maxdev, double eps_p,
@@ -149,14 +161,105 @@ __global__ void k_gram_convert(const T *__restrict__ xs, uint64_t ld, uint32_t n
         }
     }
     const double e = sqrt(E2) * (1.0 + 0x1p-30) + sqrt(X2) * 0x1p-50;
-    const double k = sigma * eps_p * 0.5 + e;
-    // cfac covers the MFMA's FP32 accumulation of the products (and the
-    // initial -W_q it accumulates onto, |W_q| <= N1_q / 2 + k_q^2: cfac
-    // holds twice the term count's roundings, and the k^2 factor the same)
-    const double w = 0.5 * (N * (1.0 - 0x1p-40) - cfac * N1 - L) - (1.0 + 0x1p-20 + cfac) * k * k;
-    W[s] = __double2float_rd(w);
+    W[s] = gram_weight(N, N1, L, e, sigma * eps_p * 0.5, cfac);
     N2[s] = N;
     if (s == 0) sigOut[0] = sigma;
+}
+
+// The scale of the projected operands: |y_c| <= sum_l |P_cl| maxdev_l.
+__global__ void k_proj_sigma(const double *__restrict__ P, uint32_t m, uint32_t d, const double *__restrict__ maxdev,
+                             int lowp, double *__restrict__ sigOut) {
+    __shared__ double red[64];
+    double v = 0.0;
+    for (uint32_t c = threadIdx.x; c < m; c += 64) {
+        double b = 0.0;
+        for (uint32_t l = 0; l < d; l++) b += fabs(P[(size_t)c * d + l]) * maxdev[l];
+        v = fmax(v, b);
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t q = 32; q > 0; q >>= 1) {
+        if (threadIdx.x < q) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + q]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double sigma = 1.0;
+        const double mx = red[0] * (1.0 + 1e-6);
+        if (lowp == 1) {
+            if (mx > 0.0) sigma = exp2(floor(log2(16384.0 / mx)));
+            sigma = fmin(fmax(sigma, 0x1p-60), 0x1p60);
+        }
+        sigOut[0] = sigma;
+    }
+}
+
+// h (and l) of y = P (x - o) on M principal axes, in double: v_l = fl(x_l -
+// o_l), y_c = FMA sum of P_cl v_l, so |y_c - P_c (x - o)| <= gamma(d + 1)
+// A_c with A_c = sum |P_cl| |v_l| (accumulated beside y_c); the 2-norm of
+// that error over the M axes, scaled by sigma, joins the measured rounding
+// e of the operands.  eps_p arrives widened by |P|_2.
+template <typename T, int LP, uint32_t M>
+__global__ __launch_bounds__(256) void k_gram_convert_proj(
+    const T *__restrict__ xs, uint64_t ld, uint32_t n, uint32_t d, const double *__restrict__ P,
+    const double *__restrict__ mean, const double *__restrict__ sigIn, double eps_p, double cfac, int split,
+    typename Lp<LP>::scalar *__restrict__ H, typename Lp<LP>::scalar *__restrict__ Lo, float *__restrict__ W,
+    double *__restrict__ N2) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double sigma = sigIn[0];
+    double y[M], A[M];
+#pragma unroll
+    for (uint32_t c = 0; c < M; c++) {
+        y[c] = 0.0;
+        A[c] = 0.0;
+    }
+    for (uint32_t l = 0; l < d; l++) {
+        const double v = (double)xs[(uint64_t)l * ld + s] - mean[l];
+        const double av = fabs(v);
+#pragma unroll
+        for (uint32_t c = 0; c < M; c++) {
+            const double pc = P[(size_t)c * d + l];
+            y[c] = fma(pc, v, y[c]);
+            A[c] = fma(fabs(pc), av, A[c]);
+        }
+    }
+    double N = 0.0, N1 = 0.0, L = 0.0, E2 = 0.0, X2 = 0.0, A2 = 0.0;
+    typename Lp<LP>::scalar *hp = H + (uint64_t)s * M;
+    typename Lp<LP>::scalar *lp = split ? Lo + (uint64_t)s * M : nullptr;
+#pragma unroll
+    for (uint32_t l0 = 0; l0 < M; l0 += 8) {
+        typename Lp<LP>::scalar hb[8], lb[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; u++) {
+            const double xp = y[l0 + u] * sigma;
+            typename Lp<LP>::scalar hv, lv = {};
+            const double hd = lp_round<LP>(xp, hv);
+            double ld2 = 0.0;
+            hb[u] = hv;
+            if (split) ld2 = lp_round<LP>(xp - hd, lv);
+            lb[u] = lv;
+            const double a = hd + ld2;
+            const double r = xp - a;
+            N += a * a;
+            N1 += (fabs(hd) + fabs(ld2)) * (fabs(hd) + fabs(ld2));
+            L += ld2 * ld2;
+            E2 += r * r;
+            X2 += xp * xp;
+            A2 += A[l0 + u] * A[l0 + u];
+        }
+        uint4 hv4, lv4;
+        __builtin_memcpy(&hv4, hb, 16);
+        *reinterpret_cast<uint4 *>(hp + l0) = hv4;
+        if (split) {
+            __builtin_memcpy(&lv4, lb, 16);
+            *reinterpret_cast<uint4 *>(lp + l0) = lv4;
+        }
+    }
+    const double gd1 = ((double)d + 2.0) * 0x1p-53 / (1.0 - ((double)d + 2.0) * 0x1p-53);
+    const double eproj = sigma * gd1 * sqrt(A2) * (1.0 + 0x1p-20);
+    const double e = sqrt(E2) * (1.0 + 0x1p-30) + sqrt(X2) * 0x1p-50 + eproj;
+    W[s] = gram_weight(N, N1, L, e, sigma * eps_p * 0.5, cfac);
+    N2[s] = N;
 }
 
 // row-major copy of the sorted points for the exact rechecks (padding zero),
@@ -214,10 +317,26 @@ static int prepare_t(GramData &gd, const Grid &G, hipStream_t s) {
     // for the FP32 rounding of W_q + W_p and of the stored N
     // (and, as the kernel accumulates onto -W_q, once more for that term)
     const double cfac = (4.0 * (gd.split ? 3.0 : 1.0) * dpad + 24.0) * 0x1p-24;
-    k_gram_convert<T, LP><<<ceil_div(n, 256), 256, 0, s>>>(
-        xs, G.ld, n, d, dpad, mean, maxdev, eps_p, cfac, gd.split, (typename Lp<LP>::scalar *)gd.h.p,
-        gd.split ? (typename Lp<LP>::scalar *)gd.l.p : nullptr, gd.w.as<float>(), gd.n2.as<double>(),
-        mean + 2 * (size_t)d);
+    auto *Ho = (typename Lp<LP>::scalar *)gd.h.p;
+    auto *Lo = gd.split ? (typename Lp<LP>::scalar *)gd.l.p : nullptr;
+    if (gd.axes) {
+        // operands on the principal axes: eps' widened by |P|_2
+        double *sig = mean + 2 * (size_t)d;
+        k_proj_sigma<<<1, 64, 0, s>>>(gd.pm.as<double>(), gd.axes, d, maxdev, LP, sig);
+        const double eps_pp = eps_p * G.gramNorm;
+        if (gd.axes == 16)
+            k_gram_convert_proj<T, LP, 16><<<ceil_div(n, 256), 256, 0, s>>>(
+                xs, G.ld, n, d, gd.pm.as<double>(), mean, sig, eps_pp, cfac, gd.split, Ho, Lo, gd.w.as<float>(),
+                gd.n2.as<double>());
+        else
+            k_gram_convert_proj<T, LP, 32><<<ceil_div(n, 256), 256, 0, s>>>(
+                xs, G.ld, n, d, gd.pm.as<double>(), mean, sig, eps_pp, cfac, gd.split, Ho, Lo, gd.w.as<float>(),
+                gd.n2.as<double>());
+    } else {
+        k_gram_convert<T, LP><<<ceil_div(n, 256), 256, 0, s>>>(xs, G.ld, n, d, dpad, mean, maxdev, eps_p, cfac,
+                                                               gd.split, Ho, Lo, gd.w.as<float>(), gd.n2.as<double>(),
+                                                               mean + 2 * (size_t)d);
+    }
     GDSJ_HIP(hipGetLastError());
     return GDSJ_OK;
 }
@@ -227,7 +346,12 @@ int gram_prepare(GramData &gd, const Grid &G, int lowp, hipStream_t s) {
         set_error("the Gram filter handles 16 to 128 dims");
         return GDSJ_EUNSUPPORTED;
     }
-    gd.dpad = pad_dims(G.d);
+    // principal axes (16 or 32) where the grid build chose them, else the dims
+    gd.axes = (G.gramAxes == 16 || G.gramAxes == 32) && G.gramAxes < pad_dims(G.d) &&
+                      G.gramP.size() == (size_t)G.gramAxes * G.d
+                  ? G.gramAxes
+                  : 0u;
+    gd.dpad = gd.axes ? gd.axes : pad_dims(G.d);
     // fp16 operands by default (one MFMA per k-step; measured faster on
     // FP32 data than bf16 hi + lo: half the operand bytes, the band only
     // slightly wider); bf16 only for FP32 data on request
@@ -235,6 +359,10 @@ int gram_prepare(GramData &gd, const Grid &G, int lowp, hipStream_t s) {
     gd.lowp = lowp;
     gd.split = lowp == 0 ? 1 : 0;  // bf16 carries 8 bits: two terms; fp16's 11 bits suffice alone
     if (G.n == 0) return GDSJ_OK;
+    if (gd.axes) {
+        GDSJ_TRY(gd.pm.ensure(G.gramP.size() * 8 + 256));
+        GDSJ_HIP(hipMemcpyAsync(gd.pm.p, G.gramP.data(), G.gramP.size() * 8, hipMemcpyHostToDevice, s));
+    }
     if (G.dtype == GDSJ_DTYPE_F32) return lowp ? prepare_t<float, 1>(gd, G, s) : prepare_t<float, 0>(gd, G, s);
     return lowp ? prepare_t<double, 1>(gd, G, s) : prepare_t<double, 0>(gd, G, s);
 }

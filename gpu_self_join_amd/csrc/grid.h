@@ -91,6 +91,15 @@ struct Grid {
     uint64_t riskyPoints = 0;
     double cellW = 0;
     DevBuf ixs, ybuf, pcaBuf;
+    // principal axes for the Gram filter's operands (set gramAxesWant before
+    // grid_build; computed only where the index's basis is): 1 the points'
+    // own dims; 0 auto (16 axes where the variance they leave out is small
+    // next to eps^2); 16 or 32: that many axes.  gramP: gramAxes x d, columns
+    // in xs's dim order; gramNorm >= the spectral norm of gramP.
+    int gramAxesWant = 1;
+    uint32_t gramAxes = 0;
+    double gramNorm = 0;
+    std::vector<double> gramP;
     DevBuf keyA, keyB, valB, tmp, aux, flag, tileKey, tileVal, cellAux, scalars;
     HostBuf hostSample;  // REORDER's sample rows, pinned (the variance is summed on the host)
     void release();

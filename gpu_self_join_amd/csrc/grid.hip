@@ -170,8 +170,8 @@ __global__ void k_cell_keys(const T *__restrict__ data, uint32_t n, uint32_t d, 
 // every accepted pair as well: both tables are then the set of pairs the
 // reference's arithmetic accepts.
 
-// covariance of the m sample rows about mean (rows row-major, d columns);
-// one block per (i, j >= i), written to both halves
+// scatter matrix of the m sample rows about mean (m times their covariance;
+// rows row-major, d columns); one block per (i, j >= i), written to both halves
 template <typename T>
 __global__ __launch_bounds__(256) void k_cov(const T *__restrict__ samp, uint32_t m, uint32_t d,
                                              const double *__restrict__ mean, double *__restrict__ cov) {
@@ -639,7 +639,10 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
     G.indexUsed = 1;
     G.riskyPoints = 0;
     const uint32_t kp = G.indexDims && G.indexDims < k ? G.indexDims : k;  // principal axes indexed
-    if (G.indexWant != 1 && n > 0 && d > k && d <= 256 && !G.sortCells) {
+    G.gramAxes = 0;
+    G.gramNorm = 0.0;
+    G.gramP.clear();
+    if ((G.indexWant != 1 || G.gramAxesWant != 1) && n > 0 && d > k && d <= 256 && !G.sortCells) {
         const uint32_t m = (n + 99) / 100;
         const size_t sb = (size_t)m * d * sizeof(T);
         if (!reorder) {
@@ -668,7 +671,33 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         // auto: when k principal axes leave at most a twentieth of the
         // variance that the reference's k dims leave out (data near a
         // rotated subspace of at most k dims)
-        pca = G.indexWant == 2 || (total - capP) < 0.05 * (total - capR);
+        pca = G.indexWant == 2 || (G.indexWant == 0 && (total - capP) < 0.05 * (total - capR));
+        // the Gram filter's operands: gm axes where the distance they leave
+        // out (twice the variance off them, for a pair) is under a hundredth
+        // of eps^2, so that they reject nearly as many pairs as all d dims
+        const uint32_t gm = G.gramAxesWant == 0 ? 16u : (uint32_t)G.gramAxesWant;
+        if (G.gramAxesWant != 1 && gm < d && d >= 32 && d <= 128) {
+            std::vector<double> P;
+            double capM = 0, totM = 0;
+            pca_basis(C, d, gm, gm, G.perm, P, &capM, &totM);
+            // (C sums the products over the m sample rows)
+            if (G.gramAxesWant != 0 || 2.0 * (totM - capM) / (double)m < 0.01 * G.eps * G.eps) {
+                // |P|_2 <= sqrt(1 + |P P^T - I|_F)
+                double f = 0.0;
+                for (uint32_t a0 = 0; a0 < gm; a0++)
+                    for (uint32_t b0 = 0; b0 < gm; b0++) {
+                        double dot = 0.0;
+                        for (uint32_t j = 0; j < d; j++) dot += P[(size_t)a0 * d + j] * P[(size_t)b0 * d + j];
+                        const double e = dot - (a0 == b0 ? 1.0 : 0.0);
+                        f += e * e;
+                    }
+                G.gramNorm = sqrt(1.0 + sqrt(f)) * (1.0 + 1e-12);
+                G.gramP.assign((size_t)gm * d, 0.0);
+                for (uint32_t c = 0; c < gm; c++)
+                    for (uint32_t l = 0; l < d; l++) G.gramP[(size_t)c * d + l] = P[(size_t)c * d + G.perm[l]];
+                G.gramAxes = gm;
+            }
+        }
         if (pca) {
             // |R|_2 <= sqrt(1 + |R R^T - I|_F)
             double f = 0.0;

@@ -133,6 +133,14 @@ typedef struct gdsj_params {
     uint32_t index_dims;    /* principal axes the grid indexes when index 2 is used (at most k;
                                0: k - 2 from k = 6, else k).  Fewer axes give larger cells: more
                                candidates per query, but more queries per tile sharing them */
+    uint32_t gram_axes;     /* the Gram filter's operands (speed only: every pair within eps' is
+                               kept and rechecked by the reference's arithmetic).  1 the points'
+                               own dims; 16 or 32: the point's coordinates on that many principal
+                               axes of the REORDER sample (32 to 128 dims, fewer axes than the
+                               padded dims, no SORT; eps' widened by the basis' spectral norm and
+                               the bound by the projection's rounding), m / d of the operand
+                               bytes and MFMAs; 0 auto: 16 axes where twice the variance they
+                               leave out is under a hundredth of eps^2, else the own dims */
 } gdsj_params;
 
 /* Timing and diagnostics of the last join on a handle. */
@@ -167,6 +175,7 @@ typedef struct gdsj_stats {
     uint64_t index_used;         /* 1 the reference's dims, 2 the principal axes (gdsj_params.index) */
     uint64_t index_risky;        /* principal axes asked for: points within rounding of a face of the
                                     reference's grid (any: the reference's dims were used) */
+    uint64_t gram_axes;          /* principal axes the Gram filter's operands were taken on (0: own dims) */
 } gdsj_stats;
 
 typedef struct gdsj_handle gdsj_handle;
