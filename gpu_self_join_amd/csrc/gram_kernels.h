@@ -175,7 +175,15 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
                 // dims per batch of loads (8 floats or 4 doubles: 8 VGPRs
                 // per row)
                 constexpr uint32_t RC = sizeof(T) == 8 ? 4u : 8u;
-                const T dv = dist_row<T, RC>(XR + (uint64_t)(q0 + r) * xrs, XR + (uint64_t)pc * xrs, a.d, mode, a.ilp);
+                // two batches in flight where the registers keep the
+                // occupancy (FP64 rows, or one k-step of fp16 operands):
+                // measured on 64-D FP64 data on 16 axes, 13.4 -> 10.8 ms per
+                // slotted pass; FP32 at two k-steps loses a wave per SIMD
+                // (9.1 -> 10.7 ms)
+                constexpr bool PF = sizeof(T) == 8 || (KS == 1 && !SPLIT);
+                const T *qrow = XR + (uint64_t)(q0 + r) * xrs, *crow = XR + (uint64_t)pc * xrs;
+                const T dv = PF ? dist_row_pf<T, RC>(qrow, crow, a.d, mode, a.ilp)
+                                : dist_row<T, RC>(qrow, crow, a.d, mode, a.ilp);
                 hit = dv <= thr;
             }
             const uint64_t hm = ballot(hit);

@@ -325,6 +325,49 @@ __device__ __forceinline__ T dist_row(const T *__restrict__ qrow, const T *__res
     return acc;
 }
 
+// dist_row with the next C dims of both rows loaded while the current C
+// are summed (two batches in flight), for rechecks whose rows come from
+// beyond L1.  Rows are padded to a multiple of 16 >= 2 C dims, so the batch
+// after l0 (l0 + C < d) stays inside the row.
+template <typename T, uint32_t C>
+__device__ __forceinline__ T dist_row_pf(const T *__restrict__ qrow, const T *__restrict__ crow, uint32_t d,
+                                         int mode, uint32_t ilp) {
+    if (mode == ACC_ILP_NOSC) return dist_row<T, C>(qrow, crow, d, mode, ilp);
+    T acc = 0;
+    uint32_t g = 0;  // position of the dim within its group
+    T cn[C], qn[C];
+#pragma unroll
+    for (uint32_t u = 0; u < C; u++) {
+        cn[u] = crow[u];
+        qn[u] = qrow[u];
+    }
+    for (uint32_t l0 = 0; l0 < d; l0 += C) {
+        T tv[C];
+#pragma unroll
+        for (uint32_t u = 0; u < C; u++) tv[u] = cn[u] - qn[u];
+        if (l0 + C < d) {
+#pragma unroll
+            for (uint32_t u = 0; u < C; u++) {
+                cn[u] = crow[l0 + C + u];
+                qn[u] = qrow[l0 + C + u];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < C; u++) {
+            if (l0 + u < d) {
+                if (mode == ACC_SEQ || g == 0) {
+                    acc = fma_rn(tv[u], tv[u], acc);
+                } else {
+                    const T p = tv[u] * tv[u];
+                    acc = acc + p;
+                }
+                g = g + 1 == ilp ? 0u : g + 1;
+            }
+        }
+    }
+    return acc;
+}
+
 // SORT=1 (kernel.cu:400-467): whether the reference evaluates candidate c
 // (sorted position cpos, in cell ccell) for query q (position qpos) at all.
 // Its cells are ordered by the value at the sort dim.
