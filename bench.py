@@ -443,13 +443,16 @@ def bench_workload(name, args, variants, steps, warmup, cpu_budget, cpu_frac, ex
                                   "unit": "T lane-ops/s", "frac": round(it / VALU_ISSUE_PEAK_TOPS, 4),
                                   "ops_per_pair": valu_ops_per_pair(d, base["ilp"], base["shortcircuit"])}}
     if gram:
-        # the MFMA Gram filter decides the pass: 2*D flops per candidate pair
-        flops = 2.0 * d * s["candidates"]
+        # the MFMA Gram filter decides the pass: 2*m flops per candidate pair,
+        # m = the operands' dims (D, or the principal axes they are taken on)
+        m = int(s.get("gram_axes", 0)) or d
+        flops = 2.0 * m * s["candidates"]
         tf = flops / (kms[dom] * 1e-3) / 1e12
         hbm = dict(roof)
         roof = {"bound": "mfma", "kernel": kname[dom], "achieved": round(tf, 3), "peak": MFMA_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(tf / MFMA_PEAK_TFLOPS, 5), "traffic": traffic,
-                "alg_flops_per_launch": flops, "candidate_pairs": s["candidates"], "avg_launch_ms": round(kms[dom], 4),
+                "alg_flops_per_launch": flops, "candidate_pairs": s["candidates"], "operand_dims": m,
+                "avg_launch_ms": round(kms[dom], 4),
                 "hbm": {k: hbm[k] for k in ("achieved", "peak", "unit", "frac", "alg_bytes_per_launch")}}
     variants_out = {}
     for vname, vr in results.items():

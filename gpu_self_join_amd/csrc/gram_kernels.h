@@ -173,8 +173,10 @@ __global__ __launch_bounds__(256) void k_gram(const T *__restrict__ xs, const ui
             if (v) {
                 nrech++;
                 // dims per batch of loads (8 floats or 4 doubles: 8 VGPRs
-                // per row)
-                constexpr uint32_t RC = sizeof(T) == 8 ? 4u : 8u;
+                // per row; 8 doubles at one k-step of operands: 10.7 ->
+                // 10.1 ms per slotted pass on 64-D FP64 data on 16 axes,
+                // at 3 waves per SIMD instead of 4)
+                constexpr uint32_t RC = sizeof(T) == 8 && KS > 1 ? 4u : 8u;
                 // two batches in flight where the registers keep the
                 // occupancy (FP64 rows, or one k-step of fp16 operands):
                 // measured on 64-D FP64 data on 16 axes, 13.4 -> 10.8 ms per
