@@ -224,22 +224,39 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_project(const T *__restrict__ data, uint32_t n, uint32_t d, uint32_t k,
                                                  const double *__restrict__ R, T *__restrict__ y,
                                                  unsigned long long *__restrict__ norm2max) {
+    // R (k x d), then the block's 256 rows, PJ dims at a time: the rows are
+    // read as runs of PJ contiguous elements instead of one element per
+    // lane 8 d bytes apart
     extern __shared__ double sR[];
+    constexpr uint32_t PJ = 8;
+    double *tile = sR + (((size_t)k * d + 1) & ~(size_t)1);  // [256][PJ + 1]
     for (uint32_t t = threadIdx.x; t < k * d; t += blockDim.x) sR[t] = R[t];
-    __syncthreads();
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t r0 = blockIdx.x * 256u;
+    const uint32_t nr = n - r0 < 256u ? n - r0 : 256u;
+    const uint32_t i = r0 + threadIdx.x;
     double nn = 0.0;
-    if (i < n) {
-        double acc[MAXK];
+    double acc[MAXK];
 #pragma unroll
-        for (int c = 0; c < MAXK; c++) acc[c] = 0.0;
-        for (uint32_t j = 0; j < d; j++) {
-            const double x = (double)data[(uint64_t)i * d + j];
-            nn = fma(x, x, nn);
-#pragma unroll
-            for (int c = 0; c < MAXK; c++)
-                if ((uint32_t)c < k) acc[c] = fma(sR[(uint32_t)c * d + j], x, acc[c]);
+    for (int c = 0; c < MAXK; c++) acc[c] = 0.0;
+    for (uint32_t j0 = 0; j0 < d; j0 += PJ) {
+        const uint32_t jc = d - j0 < PJ ? d - j0 : PJ;
+        __syncthreads();  // R is in place; the previous chunk is consumed
+        for (uint32_t e = threadIdx.x; e < 256u * PJ; e += blockDim.x) {
+            const uint32_t r = e / PJ, jj = e % PJ;
+            tile[r * (PJ + 1) + jj] = (r < nr && jj < jc) ? (double)data[(uint64_t)(r0 + r) * d + j0 + jj] : 0.0;
         }
+        __syncthreads();
+        if (threadIdx.x < nr) {
+            for (uint32_t jj = 0; jj < jc; jj++) {
+                const double x = tile[threadIdx.x * (PJ + 1) + jj];
+                nn = fma(x, x, nn);
+#pragma unroll
+                for (int c = 0; c < MAXK; c++)
+                    if ((uint32_t)c < k) acc[c] = fma(sR[(uint32_t)c * d + j0 + jj], x, acc[c]);
+            }
+        }
+    }
+    if (i < n) {
 #pragma unroll
         for (int c = 0; c < MAXK; c++)
             if ((uint32_t)c < k) y[(uint64_t)i * k + c] = (T)acc[c];
@@ -795,7 +812,8 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         T *y = G.ybuf.as<T>();
         unsigned long long *nmax = G.scalars.as<unsigned long long>() + 8;
         GDSJ_HIP(hipMemsetAsync(nmax, 0, 8, s));
-        k_project<T><<<ceil_div(n, 256), 256, (size_t)k * d * 8, s>>>(data, n, d, k, dR, y, nmax);
+        const size_t projLds = ((((size_t)k * d + 1) & ~(size_t)1) + 256 * 9) * 8;
+        k_project<T><<<ceil_div(n, 256), 256, projLds, s>>>(data, n, d, k, dR, y, nmax);
         KeyArgs ay{};
         ay.k = k;
         for (uint32_t j = 0; j < k; j++) ay.pk[j] = j;
