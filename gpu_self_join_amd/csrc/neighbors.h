@@ -340,11 +340,27 @@ __device__ __forceinline__ void tile_gaps(const T *__restrict__ xs, uint64_t ld,
         __builtin_amdgcn_wave_barrier();
         return;
     }
+    // large ranges (whole cells of tens of thousands of points): 8 loads per
+    // lane issued before their min / max, so a dim costs n / 512 memory
+    // round trips instead of n / 64 (min and max are exact in any order)
+    constexpr uint32_t U = 8;
     for (uint32_t j = 0; j < ga.k; j++) {
         double mn = 1e300, mx = -1e300;
-        for (uint32_t b = 0; b < n; b += GDSJ_WAVE) {
+        const T *col = xs + (uint64_t)j * ld + q0;
+        uint32_t b = 0;
+        for (; b + U * GDSJ_WAVE <= n; b += U * GDSJ_WAVE) {
+            T v[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) v[u] = col[b + u * GDSJ_WAVE + lane];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                mn = fmin(mn, (double)v[u]);
+                mx = fmax(mx, (double)v[u]);
+            }
+        }
+        for (; b < n; b += GDSJ_WAVE) {
             if (b + lane < n) {
-                const double v = (double)xs[(uint64_t)j * ld + q0 + b + lane];
+                const double v = (double)col[b + lane];
                 mn = fmin(mn, v);
                 mx = fmax(mx, v);
             }
