@@ -374,8 +374,9 @@ int gram_dispatch_f64(const GramData &gd, const JoinArgs &a, int mode, int pass,
 
 int gram_join_pass(const Grid &G, const GramData &gd, const JoinArgs &a, int mode, int pass,
                    unsigned long long *stats, hipStream_t s, bool measure_err) {
-    if (G.tileq > gram_tile_queries(G.d, gd.lowp)) {
-        set_error("Gram tiles hold at most %u queries here", gram_tile_queries(G.d, gd.lowp));
+    const uint32_t opd = gd.axes ? gd.axes : G.d;  // the operands' dims
+    if (G.tileq > gram_tile_queries(opd, gd.lowp)) {
+        set_error("Gram tiles hold at most %u queries here", gram_tile_queries(opd, gd.lowp));
         return GDSJ_EINVAL;
     }
     // FP32 data: fp16 or bf16 (split in two terms); FP64 data: fp16

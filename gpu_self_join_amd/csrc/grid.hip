@@ -283,10 +283,14 @@ static void jacobi_sym(std::vector<double> &A, std::vector<double> &V, uint32_t 
     V.assign((size_t)k * k, 0.0);
     for (uint32_t i = 0; i < k; i++) V[(size_t)i * k + i] = 1.0;
     for (int sweep = 0; sweep < 30; sweep++) {
-        double off = 0.0;
-        for (uint32_t i = 0; i < k; i++)
+        // converged when the off-diagonal part is at rounding level next to
+        // the diagonal (relative 1e-14 in norm)
+        double off = 0.0, dg = 0.0;
+        for (uint32_t i = 0; i < k; i++) {
+            dg += A[(size_t)i * k + i] * A[(size_t)i * k + i];
             for (uint32_t j = i + 1; j < k; j++) off += A[(size_t)i * k + j] * A[(size_t)i * k + j];
-        if (off < 1e-30) break;
+        }
+        if (off <= 1e-28 * dg || off < 1e-300) break;
         for (uint32_t p = 0; p < k; p++)
             for (uint32_t q = p + 1; q < k; q++) {
                 const double apq = A[(size_t)p * k + q];
@@ -316,11 +320,11 @@ static void jacobi_sym(std::vector<double> &A, std::vector<double> &V, uint32_t 
 // The top-k principal directions of the covariance C (d x d): subspace
 // iteration from the unit vectors of the k dims of largest variance
 // (perm[0..k)), modified Gram-Schmidt after every product, then the
-// Rayleigh-Ritz rotation that orders them by variance.  R: the first kp of
-// them (kp x d, rows orthonormal up to rounding); *capK: variance captured
-// by all k; *total: trace of C.
-static void pca_basis(const std::vector<double> &C, uint32_t d, uint32_t k, uint32_t kp, const uint32_t *perm,
-                      std::vector<double> &R, double *capK, double *total) {
+// Rayleigh-Ritz rotation that orders them by variance.  R: all k of them
+// (k x d, rows orthonormal up to rounding, by decreasing variance); ritz:
+// their variances (Ritz values, decreasing); *total: trace of C.
+static void pca_basis(const std::vector<double> &C, uint32_t d, uint32_t k, const uint32_t *perm,
+                      std::vector<double> &R, std::vector<double> &ritz, double *total) {
     std::vector<double> Q((size_t)d * k, 0.0), Z((size_t)d * k);
     for (uint32_t c = 0; c < k; c++) Q[(size_t)perm[c] * k + c] = 1.0;
     auto orth = [&](std::vector<double> &M) {
@@ -337,13 +341,18 @@ static void pca_basis(const std::vector<double> &C, uint32_t d, uint32_t k, uint
             if (nn == 0.0) M[(size_t)perm[c] * k + c] = 1.0;  // degenerate: keep the unit vector
         }
     };
+    // Out = C In, each entry summed over j in ascending order; the k
+    // columns advance together (contiguous rows of In)
     auto mulC = [&](const std::vector<double> &In, std::vector<double> &Out) {
-        for (uint32_t i = 0; i < d; i++)
-            for (uint32_t c = 0; c < k; c++) {
-                double acc = 0.0;
-                for (uint32_t j = 0; j < d; j++) acc += C[(size_t)i * d + j] * In[(size_t)j * k + c];
-                Out[(size_t)i * k + c] = acc;
+        for (uint32_t i = 0; i < d; i++) {
+            double *o = &Out[(size_t)i * k];
+            for (uint32_t c = 0; c < k; c++) o[c] = 0.0;
+            for (uint32_t j = 0; j < d; j++) {
+                const double cij = C[(size_t)i * d + j];
+                const double *in = &In[(size_t)j * k];
+                for (uint32_t c = 0; c < k; c++) o[c] += cij * in[c];
             }
+        }
     };
     for (int it = 0; it < 12; it++) {
         mulC(Q, Z);
@@ -370,10 +379,10 @@ static void pca_basis(const std::vector<double> &C, uint32_t d, uint32_t k, uint
     for (uint32_t c = 0; c < k; c++) order[c] = c;
     std::sort(order.begin(), order.end(),
               [&](uint32_t x, uint32_t y) { return A[(size_t)x * k + x] > A[(size_t)y * k + y]; });
-    *capK = 0.0;
-    for (uint32_t c = 0; c < k; c++) *capK += A[(size_t)c * k + c];
-    R.assign((size_t)kp * d, 0.0);
-    for (uint32_t c = 0; c < kp; c++)
+    ritz.resize(k);
+    for (uint32_t c = 0; c < k; c++) ritz[c] = A[(size_t)order[c] * k + order[c]];
+    R.assign((size_t)k * d, 0.0);
+    for (uint32_t c = 0; c < k; c++)
         for (uint32_t j = 0; j < d; j++) {
             double acc = 0.0;
             for (uint32_t b = 0; b < k; b++) acc += Q[(size_t)j * k + b] * V[(size_t)b * k + order[c]];
@@ -683,7 +692,12 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         GDSJ_HIP(hipMemcpyAsync(C.data(), dcov, (size_t)d * d * 8, hipMemcpyDeviceToHost, s));
         GDSJ_HIP(hipStreamSynchronize(s));
         double capP = 0, total = 0, capR = 0;
-        pca_basis(C, d, k, kp, G.perm, R, &capP, &total);
+        // the index's basis does not depend on the filter, so that every
+        // filter's rows come in the same order
+        std::vector<double> Rk, ritz;
+        pca_basis(C, d, k, G.perm, Rk, ritz, &total);
+        for (uint32_t c = 0; c < k; c++) capP += ritz[c];
+        R.assign(Rk.begin(), Rk.begin() + (size_t)kp * d);
         for (uint32_t c = 0; c < k; c++) capR += C[(size_t)G.perm[c] * d + G.perm[c]];  // the reference's dims
         // auto: when k principal axes leave at most a twentieth of the
         // variance that the reference's k dims leave out (data near a
@@ -694,11 +708,12 @@ static int grid_build_t(Grid &G, const T *data, hipStream_t s, int reorder, int 
         // of eps^2, so that they reject nearly as many pairs as all d dims
         const uint32_t gm = G.gramAxesWant == 0 ? 16u : (uint32_t)G.gramAxesWant;
         if (G.gramAxesWant != 1 && gm < d && d >= 32 && d <= 128) {
-            std::vector<double> P;
-            double capM = 0, totM = 0;
-            pca_basis(C, d, gm, gm, G.perm, P, &capM, &totM);
+            std::vector<double> P, ritzM;
+            double capM = 0;
+            pca_basis(C, d, gm, G.perm, P, ritzM, &total);
+            for (uint32_t c = 0; c < gm; c++) capM += ritzM[c];
             // (C sums the products over the m sample rows)
-            if (G.gramAxesWant != 0 || 2.0 * (totM - capM) / (double)m < 0.01 * G.eps * G.eps) {
+            if (G.gramAxesWant != 0 || 2.0 * (total - capM) / (double)m < 0.01 * G.eps * G.eps) {
                 // |P|_2 <= sqrt(1 + |P P^T - I|_F)
                 double f = 0.0;
                 for (uint32_t a0 = 0; a0 < gm; a0++)
